@@ -1,0 +1,147 @@
+/*
+ * pond.h -- C ABI of the MI355X-native particles-on-demand (PonD) step.
+ *
+ * This is the drop-in boundary for the reference's step API (engrab6/lbmPonD):
+ *
+ *   reference                                         here
+ *   ------------------------------------------------  ---------------------------------------
+ *   LBMParamsHost::set / PhysPars::set                pond_params_default, pond_create
+ *     (lbm.cu:134-168, structs.cuh:96-149)
+ *   init() + fill<<<>>>  (init.cu:96-128)             pond_set_state (set_base per cell)
+ *   copy2dev / Group::pack of a full state            pond_set_populations
+ *     (params.h:136-146, structs.cuh:269-288)
+ *   calcStep()  (update.cu:86-142, minus I/O)         pond_step
+ *   PonDstep / PonDstepD3Q125 kernels                 (launched inside pond_step)
+ *     (update.cu:11-14, steps.cu:151-250)
+ *   drop(std::string*)  (drop.cu:2-34)                pond_drop (same .arr format)
+ *   diagnostics() + calcTotMoments + NiterMax         pond_diagnostics
+ *     (diagn.cu:24-30, update.cu:52-69,124-135, steps.cu:513)
+ *   reading loadGroups back (drop.cu:3-4)             pond_get_fields, pond_get_populations
+ *   CHECK_ERROR -> exit  (params.h:74-81)             pond_status codes + pond_last_error
+ *   global parsHost / pars singletons (lbm.cu:14-15)  opaque pond_ctx, one per simulation
+ *
+ * All arrays are host arrays in the reference's cell order: linear index x + Nx*(y + Ny*z)
+ * (structs.cuh:153-166).  Population arrays are plane-major: f[i*N + cell].
+ * A context owns its device memory and one HIP stream; it is not thread-safe.
+ */
+#ifndef POND_H
+#define POND_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  POND_OK = 0,
+  POND_ERR_INVALID = 1, /* bad argument or unsupported combination */
+  POND_ERR_HIP = 2,     /* HIP runtime error (launch, memcpy, fault) */
+  POND_ERR_NOMEM = 3,   /* device or host allocation failed */
+  POND_ERR_IO = 4,      /* file I/O failed (pond_drop, pond_read_arr) */
+  POND_ERR_STATE = 5    /* call out of order (e.g. step before any state was set) */
+} pond_status;
+
+/* lattices: consts.h:111-134 (D1Q5), 14-65 (D3Q27), 136-245 (D3Q125) */
+typedef enum { POND_D1Q5 = 0, POND_D3Q27 = 1, POND_D3Q125 = 2 } pond_lattice;
+
+/* interpolators for D1Q5/D3Q27; D3Q125 always uses its fixed 3x3x3 Lagrange stencil
+   (steps.cu:204-212).
+   POND_WENO5  propagateWENO<3,5> as written (steps.cu:458, 774-825), including its y/z
+               sub-stencil at floor(y+0.5)+0..2 and smoothness row (r mod Ny, r mod Nz).  The
+               y/z stencil extrapolates downwind, so in 3D grid-scale modes grow every step.
+   POND_LAGR4  propagate<4>, the reference's active interpolator (steps.cu:451, 986-1036).
+   POND_WENO5C the corrected WENO5 (no reference counterpart): y/z nodes floor(y+0.5)-1..+1,
+               smoothness row through the departure point.  Identical to POND_WENO5 on a line
+               (Ny = Nz = 1). */
+typedef enum { POND_WENO5 = 0, POND_LAGR4 = 1, POND_WENO5C = 2 } pond_interp;
+
+/* POND_FP64: plain path, reference operation order, FP64 everywhere.
+   POND_SPLIT: FP32 populations/streaming/interpolation/collide, compensated moment sums,
+   FP64 gauge residual, temperature/energy moment and moment solve. */
+typedef enum { POND_FP64 = 0, POND_SPLIT = 1 } pond_precision;
+
+typedef struct {
+  int lattice;      /* pond_lattice */
+  int interp;       /* pond_interp */
+  int precision;    /* pond_precision */
+  int64_t nx, ny, nz;
+  double dtau;      /* BGK relaxation (PhysPars::set: 1/0.9) */
+  double tol_abs;   /* isConv absolute tolerance (steps.cu:62: 1e-12) */
+  double tol_rel;   /* isConv relative tolerance (steps.cu:63: 1e-10) */
+  int max_iter;     /* fixed-point cap (steps.cu:422: 100) */
+  int use_mlp;      /* 1: learned starting gauge (POND_SPLIT only) */
+  double split_tol_rel; /* POND_SPLIT: relative tolerance on the FP64 gauge residual */
+  double split_tol_abs; /* POND_SPLIT: absolute tolerance on the FP64 gauge residual */
+} pond_params;
+
+typedef struct {
+  double mass;       /* sum rho                       (update.cu:65) */
+  double momentum_x; /* sum rho*ux                    (update.cu:66) */
+  double energy;     /* sum rho*(|u|^2 + Dim*T)       (update.cu:67) */
+  double k;          /* kinetic energy of diagn.cu:18 */
+  int32_t niter_max; /* max gauge iterations, last step (steps.cu:513) */
+  double niter_mean; /* mean gauge iterations per cell, last step */
+  int64_t step;      /* steps taken (LBMParams::iStep) */
+} pond_diag;
+
+typedef struct {
+  double last_ms;    /* device time of the last pond_step call (HIP events on the ctx stream) */
+  double total_ms;   /* accumulated device time of all pond_step calls */
+  int64_t steps;     /* steps covered by total_ms */
+  double last_mlp_ms;/* part of last_ms spent in the learned initializer */
+} pond_timing;
+
+typedef struct pond_ctx pond_ctx;
+
+/* defaults of the reference: dtau=1/0.9, tol 1e-12 + 1e-10*|x|, cap 100, WENO5, FP64 */
+void pond_params_default(pond_params* p, int lattice);
+
+pond_status pond_create(const pond_params* p, pond_ctx** out);
+void pond_destroy(pond_ctx* ctx);
+
+/* last error message of ctx (or of the last failed pond_create when ctx is NULL) */
+const char* pond_last_error(const pond_ctx* ctx);
+
+/* number of populations of a lattice (5, 27, 125); 0 for an unknown lattice */
+int pond_qn(int lattice);
+
+/* lattice tables as evaluated by this library: e (int, Qn*3), ef (Qn*3), w (Qn),
+   tlat_dim[2] = {TLat, Dim} */
+pond_status pond_lattice_tables(int lattice, int32_t* e, double* ef, double* w, double* tlat_dim);
+
+/* equilibrium state f_i = w_i*rho in each cell's own gauge (Cell::set_base, lbpack.cu:15-21) */
+pond_status pond_set_state(pond_ctx* ctx, const double* rho, const double* ux, const double* uy,
+                           const double* uz, const double* T);
+
+/* arbitrary full state: f plane-major (Qn*N) plus the five moment planes */
+pond_status pond_set_populations(pond_ctx* ctx, const double* f, const double* rho, const double* ux,
+                                 const double* uy, const double* uz, const double* T);
+
+/* nsteps PonD steps (calcStep without I/O).  Asynchronous on the ctx stream; returns after
+   the launches are queued.  pond_sync / any get call waits. */
+pond_status pond_step(pond_ctx* ctx, int nsteps);
+pond_status pond_sync(pond_ctx* ctx);
+
+/* current state (output of the last step); any pointer may be NULL */
+pond_status pond_get_fields(pond_ctx* ctx, double* rho, double* ux, double* uy, double* uz, double* T,
+                            int32_t* niter);
+pond_status pond_get_populations(pond_ctx* ctx, double* f);
+
+pond_status pond_diagnostics(pond_ctx* ctx, pond_diag* out);
+pond_status pond_timing_info(pond_ctx* ctx, pond_timing* out);
+
+/* write Rho/Vx/Vy/Vz/T_%07d.arr of the current state into directory dir (drop.cu:2-34) */
+pond_status pond_drop(pond_ctx* ctx, const char* dir);
+
+/* read one .arr file written by pond_drop (or by the reference's drop()) into out
+   (nx*ny*nz doubles); dims[3] receives Nx,Ny,Nz.  out may be NULL to query dims. */
+pond_status pond_read_arr(const char* path, int64_t* dims, double* out);
+
+/* device bytes held by the ctx (state buffers, scratch, network) */
+int64_t pond_device_bytes(const pond_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* POND_H */

@@ -1,0 +1,454 @@
+// pond_api.hip -- C ABI of the PonD step (include/pond.h): context lifetime, state upload and
+// download, the step driver (calcStep, update.cu:86-142, without the I/O it times), global
+// diagnostics and the .arr field dump.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "pond_ctx.h"
+
+using namespace pond;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+#define POND_HIP(ctx, expr)                                                                  \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) {                                                                  \
+      (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                        \
+      return POND_ERR_HIP;                                                                   \
+    }                                                                                        \
+  } while (0)
+
+pond_status fail(pond_ctx* ctx, pond_status s, const std::string& msg) {
+  ctx->err = msg;
+  return s;
+}
+
+int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
+
+// ---------------------------------------------------------------------------------- kernels
+
+// Cell::set_base (lbpack.cu:15-21) from the moment planes into the population planes
+template <typename FT>
+__global__ void k_set_base(FT* __restrict__ f, const double* __restrict__ rho, int64_t n, int64_t plane, int qn,
+                           const StepArgs a) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const double r = rho[c];
+  for (int i = 0; i < qn; i++) f[(int64_t)i * plane + c] = (FT)(a.w[i] * r);
+}
+
+// block-level deterministic partial sums: mass, momentum_x, energy, k, niter sum; niter max
+struct RedOut {
+  double s[5];
+  int imax;
+};
+
+__device__ double block_sum(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) t += sh[i];
+  return t;
+}
+
+__device__ int block_max(int v, int* sh) {
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  int t = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) t = max(t, sh[i]);
+  return t;
+}
+
+// calcTotMoments (update.cu:52-69) + diagn (diagn.cu:3-22) + NiterMax, for the FP64 state
+__global__ __launch_bounds__(256) void k_reduce_f64(const double* __restrict__ m, const int* __restrict__ niter,
+                                                    int64_t n, int64_t plane, double dim, double cx, double cy,
+                                                    double cz, RedOut* __restrict__ out) {
+  __shared__ double shd[4];
+  __shared__ int shi[4];
+  double s[5] = {0, 0, 0, 0, 0};
+  int imax = 0;
+  const double d8pi3 = 1.0 / (8 * M_PI * M_PI * M_PI);
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+    const double rho = m[c], ux = m[plane + c], uy = m[2 * plane + c], uz = m[3 * plane + c], T = m[4 * plane + c];
+    s[0] += rho * 1.0;
+    s[1] += rho * ux * 1.0;
+    s[2] += rho * (ux * ux + uy * uy + uz * uz + dim * T) * 1.0;
+    s[3] += 0.5 * d8pi3 * (ux * ux + uy * uy + uz * uz) * cx * cy * cz;
+    s[4] += (double)niter[c];
+    imax = max(imax, niter[c]);
+  }
+  RedOut r;
+  for (int q = 0; q < 5; q++) r.s[q] = block_sum(s[q], shd);
+  r.imax = block_max(imax, shi);
+  if (threadIdx.x == 0) out[blockIdx.x] = r;
+}
+
+__global__ void k_reduce_final(const RedOut* __restrict__ part, int nb, RedOut* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  RedOut r{};
+  for (int b = 0; b < nb; b++) {
+    for (int q = 0; q < 5; q++) r.s[q] += part[b].s[q];
+    r.imax = max(r.imax, part[b].imax);
+  }
+  *out = r;
+}
+
+constexpr int kRedBlocks = 1024;
+
+pond_status ensure_events(pond_ctx* ctx) {
+  if (ctx->ev_pool.empty()) {
+    for (int i = 0; i < 64; i++) {
+      EventPair e;
+      POND_HIP(ctx, hipEventCreate(&e.a));
+      POND_HIP(ctx, hipEventCreate(&e.b));
+      ctx->ev_pool.push_back(e);
+    }
+  }
+  return POND_OK;
+}
+
+// fold the finished event pairs into the timing totals
+pond_status collect_events(pond_ctx* ctx) {
+  if (ctx->ev_pending.empty()) return POND_OK;
+  double ms_sum = 0;
+  for (auto& e : ctx->ev_pending) {
+    POND_HIP(ctx, hipEventSynchronize(e.b));
+    float ms = 0;
+    POND_HIP(ctx, hipEventElapsedTime(&ms, e.a, e.b));
+    ms_sum += ms;
+    ctx->ev_pool.push_back(e);
+  }
+  ctx->last_ms = ms_sum;
+  ctx->total_ms += ms_sum;
+  ctx->timed_steps += (int64_t)ctx->ev_pending.size();
+  ctx->ev_pending.clear();
+  return POND_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void pond_params_default(pond_params* p, int lattice) {
+  std::memset(p, 0, sizeof(*p));
+  p->lattice = lattice;
+  p->interp = POND_WENO5;
+  p->precision = POND_FP64;
+  p->nx = p->ny = p->nz = 1;
+  p->dtau = 1.0 / 0.9;  // PhysPars::set (structs.cuh:98)
+  p->tol_abs = 1e-12;   // isConv (steps.cu:62-63)
+  p->tol_rel = 1e-10;
+  p->max_iter = 100;    // steps.cu:422
+  p->use_mlp = 0;
+  p->split_tol_rel = 1e-6;
+  p->split_tol_abs = 1e-9;
+}
+
+int pond_qn(int lattice) { return qn_of(lattice); }
+
+pond_status pond_lattice_tables(int lattice, int32_t* e, double* ef, double* w, double* tlat_dim) {
+  LatticeTables t;
+  if (!make_lattice(lattice, &t)) return POND_ERR_INVALID;
+  for (int i = 0; i < t.Qn; i++) {
+    for (int d = 0; d < 3; d++) {
+      if (e) e[3 * i + d] = t.e[i][d];
+      if (ef) ef[3 * i + d] = t.ef[i][d];
+    }
+    if (w) w[i] = t.w[i];
+  }
+  if (tlat_dim) { tlat_dim[0] = t.TLat; tlat_dim[1] = t.Dim; }
+  return POND_OK;
+}
+
+const char* pond_last_error(const pond_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+pond_status pond_create(const pond_params* p, pond_ctx** out) {
+  if (!p || !out) { g_create_error = "pond_create: null argument"; return POND_ERR_INVALID; }
+  *out = nullptr;
+  auto* ctx = new (std::nothrow) pond_ctx();
+  if (!ctx) { g_create_error = "pond_create: out of host memory"; return POND_ERR_NOMEM; }
+  auto bail = [&](pond_status s, const std::string& m) {
+    g_create_error = m;
+    pond_destroy(ctx);
+    return s;
+  };
+  ctx->p = *p;
+  if (!make_lattice(p->lattice, &ctx->L)) return bail(POND_ERR_INVALID, "pond_create: unknown lattice");
+  if (p->nx < 1 || p->ny < 1 || p->nz < 1 || p->nx > (1 << 20) || p->ny > (1 << 20) || p->nz > (1 << 20))
+    return bail(POND_ERR_INVALID, "pond_create: grid sizes must be in [1, 2^20]");
+  if (p->interp != POND_WENO5 && p->interp != POND_LAGR4 && p->interp != POND_WENO5C) return bail(POND_ERR_INVALID, "pond_create: unknown interp");
+  if (p->precision != POND_FP64 && p->precision != POND_SPLIT)
+    return bail(POND_ERR_INVALID, "pond_create: unknown precision");
+  if (p->precision == POND_SPLIT) return bail(POND_ERR_INVALID, "pond_create: POND_SPLIT is not built in this library");
+  if (p->max_iter < 1) return bail(POND_ERR_INVALID, "pond_create: max_iter must be >= 1");
+  if (p->use_mlp && p->precision != POND_SPLIT)
+    return bail(POND_ERR_INVALID, "pond_create: the learned initializer requires POND_SPLIT");
+  ctx->Qn = ctx->L.Qn;
+  Grid& g = ctx->g;
+  g.nx = (int)p->nx; g.ny = (int)p->ny; g.nz = (int)p->nz;
+  g.n = p->nx * p->ny * p->nz;
+  g.plane = round_up(g.n, 64);
+  StepArgs& a = ctx->args;
+  a.g = g;
+  a.dtau = p->dtau;
+  a.tol_abs = p->tol_abs;
+  a.tol_rel = p->tol_rel;
+  a.max_iter = p->max_iter;
+  a.sqrt_tbase = std::sqrt(ctx->L.TLat);
+  a.tlat = ctx->L.TLat;
+  a.inv_dim = 1. / ctx->L.Dim;
+  for (int i = 0; i < ctx->Qn; i++) a.w[i] = ctx->L.w[i];
+
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(POND_ERR_HIP, "pond_create: hipStreamCreate failed");
+  const size_t fbytes = (size_t)ctx->Qn * g.plane * sizeof(double);
+  const size_t mbytes = (size_t)5 * g.plane * sizeof(double);
+  for (int b = 0; b < 2; b++) {
+    if (hipMalloc(&ctx->f64[b].f, fbytes) != hipSuccess || hipMalloc(&ctx->f64[b].m, mbytes) != hipSuccess)
+      return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of the state buffers failed");
+    ctx->bytes += (int64_t)(fbytes + mbytes);
+  }
+  if (hipMalloc(&ctx->niter, (size_t)g.plane * sizeof(int)) != hipSuccess)
+    return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of niter failed");
+  ctx->red_len = kRedBlocks + 1;
+  if (hipMalloc(&ctx->red, (size_t)ctx->red_len * sizeof(RedOut)) != hipSuccess)
+    return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of the reduction buffer failed");
+  ctx->bytes += (int64_t)g.plane * sizeof(int) + ctx->red_len * (int64_t)sizeof(RedOut);
+  if (hipMemsetAsync(ctx->niter, 0, (size_t)g.plane * sizeof(int), ctx->stream) != hipSuccess)
+    return bail(POND_ERR_HIP, "pond_create: hipMemset failed");
+  if (ensure_events(ctx) != POND_OK) return bail(POND_ERR_HIP, "pond_create: " + ctx->err);
+  *out = ctx;
+  return POND_OK;
+}
+
+void pond_destroy(pond_ctx* ctx) {
+  if (!ctx) return;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (int b = 0; b < 2; b++) {
+    if (ctx->f64[b].f) (void)hipFree(ctx->f64[b].f);
+    if (ctx->f64[b].m) (void)hipFree(ctx->f64[b].m);
+    if (ctx->sp[b].f) (void)hipFree(ctx->sp[b].f);
+    if (ctx->sp[b].m) (void)hipFree(ctx->sp[b].m);
+    if (ctx->sp[b].T) (void)hipFree(ctx->sp[b].T);
+  }
+  if (ctx->niter) (void)hipFree(ctx->niter);
+  if (ctx->red) (void)hipFree(ctx->red);
+  for (auto& e : ctx->ev_pool) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+  for (auto& e : ctx->ev_pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+static pond_status upload_moments(pond_ctx* ctx, double* mdev, const double* rho, const double* ux, const double* uy,
+                                  const double* uz, const double* T) {
+  const double* src[5] = {rho, ux, uy, uz, T};
+  for (int k = 0; k < 5; k++) {
+    if (!src[k]) return fail(ctx, POND_ERR_INVALID, "null field pointer");
+    POND_HIP(ctx, hipMemcpyAsync(mdev + (size_t)k * ctx->g.plane, src[k], (size_t)ctx->g.n * sizeof(double),
+                                 hipMemcpyHostToDevice, ctx->stream));
+  }
+  return POND_OK;
+}
+
+pond_status pond_set_state(pond_ctx* ctx, const double* rho, const double* ux, const double* uy, const double* uz,
+                           const double* T) {
+  if (!ctx) return POND_ERR_INVALID;
+  const Grid& g = ctx->g;
+  pond_status s = upload_moments(ctx, ctx->f64[0].m, rho, ux, uy, uz, T);
+  if (s != POND_OK) return s;
+  const unsigned blocks = (unsigned)((g.n + 255) / 256);
+  hipLaunchKernelGGL(k_set_base<double>, dim3(blocks), dim3(256), 0, ctx->stream, ctx->f64[0].f, ctx->f64[0].m, g.n,
+                     g.plane, ctx->Qn, ctx->args);
+  POND_HIP(ctx, hipGetLastError());
+  // both buffers hold the initial state, as fill() does (init.cu:104-105)
+  POND_HIP(ctx, hipMemcpyAsync(ctx->f64[1].f, ctx->f64[0].f, (size_t)ctx->Qn * g.plane * sizeof(double),
+                               hipMemcpyDeviceToDevice, ctx->stream));
+  POND_HIP(ctx, hipMemcpyAsync(ctx->f64[1].m, ctx->f64[0].m, (size_t)5 * g.plane * sizeof(double),
+                               hipMemcpyDeviceToDevice, ctx->stream));
+  POND_HIP(ctx, hipMemsetAsync(ctx->niter, 0, (size_t)g.plane * sizeof(int), ctx->stream));
+  POND_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->cur = 0;
+  ctx->has_state = true;
+  ctx->step = 0;
+  return POND_OK;
+}
+
+pond_status pond_set_populations(pond_ctx* ctx, const double* f, const double* rho, const double* ux, const double* uy,
+                                 const double* uz, const double* T) {
+  if (!ctx) return POND_ERR_INVALID;
+  if (!f) return fail(ctx, POND_ERR_INVALID, "pond_set_populations: null f");
+  const Grid& g = ctx->g;
+  for (int b = 0; b < 2; b++) {
+    pond_status s = upload_moments(ctx, ctx->f64[b].m, rho, ux, uy, uz, T);
+    if (s != POND_OK) return s;
+    for (int i = 0; i < ctx->Qn; i++)
+      POND_HIP(ctx, hipMemcpyAsync(ctx->f64[b].f + (size_t)i * g.plane, f + (size_t)i * g.n,
+                                   (size_t)g.n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  }
+  POND_HIP(ctx, hipMemsetAsync(ctx->niter, 0, (size_t)g.plane * sizeof(int), ctx->stream));
+  POND_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->cur = 0;
+  ctx->has_state = true;
+  ctx->step = 0;
+  return POND_OK;
+}
+
+pond_status pond_step(pond_ctx* ctx, int nsteps) {
+  if (!ctx) return POND_ERR_INVALID;
+  if (!ctx->has_state) return fail(ctx, POND_ERR_STATE, "pond_step: no state set");
+  if (nsteps < 0) return fail(ctx, POND_ERR_INVALID, "pond_step: nsteps < 0");
+  pond_status s = collect_events(ctx);
+  if (s != POND_OK) return s;
+  for (int k = 0; k < nsteps; k++) {
+    if (ctx->ev_pool.empty()) {
+      s = collect_events(ctx);
+      if (s != POND_OK) return s;
+    }
+    EventPair e = ctx->ev_pool.back();
+    ctx->ev_pool.pop_back();
+    const int in = ctx->cur, outb = 1 - ctx->cur;
+    POND_HIP(ctx, hipEventRecord(e.a, ctx->stream));
+    hipError_t he = launch_step_f64(ctx->p.lattice, ctx->p.interp, ctx->f64[in].f, ctx->f64[in].m, ctx->f64[outb].f,
+                                    ctx->f64[outb].m, ctx->niter, ctx->args, ctx->stream);
+    if (he != hipSuccess) return fail(ctx, POND_ERR_HIP, std::string("step kernel launch: ") + hipGetErrorString(he));
+    POND_HIP(ctx, hipEventRecord(e.b, ctx->stream));
+    ctx->ev_pending.push_back(e);
+    ctx->cur = outb;  // swapLS (structs.cuh:148)
+    ctx->step++;
+  }
+  return POND_OK;
+}
+
+pond_status pond_sync(pond_ctx* ctx) {
+  if (!ctx) return POND_ERR_INVALID;
+  POND_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return collect_events(ctx);
+}
+
+pond_status pond_get_fields(pond_ctx* ctx, double* rho, double* ux, double* uy, double* uz, double* T, int32_t* niter) {
+  if (!ctx) return POND_ERR_INVALID;
+  if (!ctx->has_state) return fail(ctx, POND_ERR_STATE, "pond_get_fields: no state set");
+  pond_status s = pond_sync(ctx);
+  if (s != POND_OK) return s;
+  double* dst[5] = {rho, ux, uy, uz, T};
+  const Grid& g = ctx->g;
+  for (int k = 0; k < 5; k++)
+    if (dst[k])
+      POND_HIP(ctx, hipMemcpy(dst[k], ctx->f64[ctx->cur].m + (size_t)k * g.plane, (size_t)g.n * sizeof(double),
+                              hipMemcpyDeviceToHost));
+  if (niter) POND_HIP(ctx, hipMemcpy(niter, ctx->niter, (size_t)g.n * sizeof(int), hipMemcpyDeviceToHost));
+  return POND_OK;
+}
+
+pond_status pond_get_populations(pond_ctx* ctx, double* f) {
+  if (!ctx) return POND_ERR_INVALID;
+  if (!ctx->has_state) return fail(ctx, POND_ERR_STATE, "pond_get_populations: no state set");
+  if (!f) return fail(ctx, POND_ERR_INVALID, "pond_get_populations: null f");
+  pond_status s = pond_sync(ctx);
+  if (s != POND_OK) return s;
+  const Grid& g = ctx->g;
+  for (int i = 0; i < ctx->Qn; i++)
+    POND_HIP(ctx, hipMemcpy(f + (size_t)i * g.n, ctx->f64[ctx->cur].f + (size_t)i * g.plane,
+                            (size_t)g.n * sizeof(double), hipMemcpyDeviceToHost));
+  return POND_OK;
+}
+
+pond_status pond_diagnostics(pond_ctx* ctx, pond_diag* out) {
+  if (!ctx || !out) return POND_ERR_INVALID;
+  if (!ctx->has_state) return fail(ctx, POND_ERR_STATE, "pond_diagnostics: no state set");
+  const Grid& g = ctx->g;
+  auto* part = reinterpret_cast<RedOut*>(ctx->red);
+  const int nb = (int)std::min<int64_t>(kRedBlocks, (g.n + 255) / 256);
+  hipLaunchKernelGGL(k_reduce_f64, dim3(nb), dim3(256), 0, ctx->stream, ctx->f64[ctx->cur].m, ctx->niter, g.n, g.plane,
+                     ctx->L.Dim, 2 * M_PI / g.nx, 2 * M_PI / g.ny, 2 * M_PI / g.nz, part);
+  POND_HIP(ctx, hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, ctx->stream, part, nb, part + kRedBlocks);
+  POND_HIP(ctx, hipGetLastError());
+  RedOut r;
+  POND_HIP(ctx, hipMemcpyAsync(&r, part + kRedBlocks, sizeof(RedOut), hipMemcpyDeviceToHost, ctx->stream));
+  pond_status s = pond_sync(ctx);
+  if (s != POND_OK) return s;
+  out->mass = r.s[0];
+  out->momentum_x = r.s[1];
+  out->energy = r.s[2];
+  out->k = r.s[3];
+  out->niter_mean = ctx->step > 0 ? r.s[4] / (double)g.n : 0.0;
+  out->niter_max = ctx->step > 0 ? r.imax : 0;
+  out->step = ctx->step;
+  return POND_OK;
+}
+
+pond_status pond_timing_info(pond_ctx* ctx, pond_timing* out) {
+  if (!ctx || !out) return POND_ERR_INVALID;
+  pond_status s = pond_sync(ctx);
+  if (s != POND_OK) return s;
+  out->last_ms = ctx->last_ms;
+  out->total_ms = ctx->total_ms;
+  out->steps = ctx->timed_steps;
+  out->last_mlp_ms = ctx->last_mlp_ms;
+  return POND_OK;
+}
+
+int64_t pond_device_bytes(const pond_ctx* ctx) { return ctx ? ctx->bytes : 0; }
+
+// drop() (drop.cu:2-34): five files Rho/Vx/Vy/Vz/T_%07d.arr, header
+// {int comment_len=0, int dim=3, int sizeof(double), int Nx, int Ny, int Nz}, then the field
+// z-outer / x-fastest
+pond_status pond_drop(pond_ctx* ctx, const char* dir) {
+  if (!ctx || !dir) return POND_ERR_INVALID;
+  const Grid& g = ctx->g;
+  std::vector<double> buf((size_t)g.n);
+  static const char* names[5] = {"Rho", "Vx", "Vy", "Vz", "T"};
+  for (int k = 0; k < 5; k++) {
+    double* dst[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    dst[k] = buf.data();
+    pond_status s = pond_get_fields(ctx, dst[0], dst[1], dst[2], dst[3], dst[4], nullptr);
+    if (s != POND_OK) return s;
+    char path[4096];
+    std::snprintf(path, sizeof(path), "%s/%s_%07d.arr", dir, names[k], (int)ctx->step);
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) return fail(ctx, POND_ERR_IO, std::string("pond_drop: cannot open ") + path);
+    const int32_t hdr[6] = {0, 3, (int32_t)sizeof(double), g.nx, g.ny, g.nz};
+    bool ok = std::fwrite(hdr, sizeof(int32_t), 6, fp) == 6;
+    ok = ok && std::fwrite(buf.data(), sizeof(double), (size_t)g.n, fp) == (size_t)g.n;
+    ok = (std::fclose(fp) == 0) && ok;
+    if (!ok) return fail(ctx, POND_ERR_IO, std::string("pond_drop: write failed: ") + path);
+  }
+  return POND_OK;
+}
+
+pond_status pond_read_arr(const char* path, int64_t* dims, double* out) {
+  if (!path) return POND_ERR_INVALID;
+  FILE* fp = std::fopen(path, "rb");
+  if (!fp) return POND_ERR_IO;
+  int32_t clen = 0;
+  pond_status st = POND_OK;
+  if (std::fread(&clen, sizeof(int32_t), 1, fp) != 1 || clen < 0 || clen % 4 != 0) st = POND_ERR_IO;
+  if (st == POND_OK && clen > 0 && std::fseek(fp, clen, SEEK_CUR) != 0) st = POND_ERR_IO;
+  int32_t h[5] = {0, 0, 0, 0, 0};
+  if (st == POND_OK && std::fread(h, sizeof(int32_t), 5, fp) != 5) st = POND_ERR_IO;
+  if (st == POND_OK && (h[0] != 3 || h[1] != (int32_t)sizeof(double) || h[2] < 1 || h[3] < 1 || h[4] < 1))
+    st = POND_ERR_IO;
+  if (st == POND_OK) {
+    if (dims) { dims[0] = h[2]; dims[1] = h[3]; dims[2] = h[4]; }
+    const size_t n = (size_t)h[2] * h[3] * h[4];
+    if (out && std::fread(out, sizeof(double), n, fp) != n) st = POND_ERR_IO;
+  }
+  std::fclose(fp);
+  return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,68 @@
+// pond_device.h -- device-side helpers and kernel argument structs shared by the step kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "lattice.h"
+
+namespace pond {
+
+// Cell-grid geometry.  Cells are stored x fastest (structs.cuh:153-166); every field is a
+// plane of `plane` elements (plane >= n, padded so each plane starts 256-byte aligned).
+struct Grid {
+  int nx, ny, nz;
+  int64_t n;
+  int64_t plane;
+};
+
+// Parameters of one step, passed by value to the kernels.
+struct StepArgs {
+  Grid g;
+  double dtau;
+  double tol_abs, tol_rel;
+  int max_iter;
+  double sqrt_tbase;  // sqrt(TLat), evaluated once on the host (structs.cuh:530)
+  double tlat;
+  double inv_dim;     // 1./Dim (structs.cuh:330)
+  double w[kQmax];    // lattice weights for the collide and set_base
+};
+
+// per-step counters written by the step kernels (one atomic per wave)
+struct StepStats {
+  unsigned long long niter_sum;
+  int niter_max;
+  int pad;
+};
+
+__device__ __forceinline__ int isign_d(double v) { return (0.0 < v) - (v < 0.0); }
+
+// params.h:110-112
+__device__ __forceinline__ double sqrts_d(double v) { return sqrt(fabs(v)) * (double)isign_d(v); }
+
+__device__ __forceinline__ bool finite_d(double v) { return !__builtin_isnan(v) && !__builtin_isinf(v); }
+
+// periodic wrap, exactly (v % n + n) % n for any v (steps.cu:1015-1017)
+__device__ __forceinline__ int wrapi(int v, int n) {
+  if (v >= 0 && v < n) return v;
+  int r = v % n;
+  return r < 0 ? r + n : r;
+}
+
+__device__ __forceinline__ int64_t cell_index(const Grid& g, int x, int y, int z) {
+  return (int64_t)x + (int64_t)g.nx * ((int64_t)y + (int64_t)g.ny * (int64_t)z);
+}
+
+// wave-level reductions for the per-step counters
+__device__ __forceinline__ void stats_add(StepStats* st, int niter, bool active) {
+  unsigned long long v = active ? (unsigned long long)niter : 0ull;
+  int m = active ? niter : 0;
+  for (int off = 32; off > 0; off >>= 1) {
+    v += __shfl_xor(v, off, 64);
+    m = max(m, __shfl_xor(m, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&st->niter_sum, v);
+    atomicMax(&st->niter_max, m);
+  }
+}
+
+}  // namespace pond

@@ -1,0 +1,593 @@
+// step_f64.hip -- plain FP64 PonD step for gfx950, reference operation order.
+//
+// Replaces PonDstep -> LBMpull<> (update.cu:11-14, steps.cu:356-588) and PonDstepD3Q125
+// (steps.cu:151-250).  Every rounding step of the reference is reproduced: loops run in the
+// reference order, expression trees are kept, the build disables FMA contraction.  The only
+// changes are ones that cannot alter a bit:
+//   * pure sub-expressions are hoisted (the node's sqrt(|T0|)*sign(T0), the per-target
+//     constants, the shared left prefixes W*gx, (W*gx)*gy of the left-associated products);
+//   * a node's transfer to a given target is computed once and reused where the reference
+//     recomputes it (the three overlapping WENO sub-stencils; 1D grids, where all y/z stencil
+//     nodes are the same cell);
+//   * populations of the iterate live in the output buffer instead of a register record, and
+//     the moment sums (structs.cuh:323-327) are accumulated while the populations are produced,
+//     in the same population order.
+//
+// D1Q5/D3Q27 mapping: one lane per cell, 64 consecutive cells (x fastest) per wave, so node
+// loads of neighbouring lanes are neighbouring addresses of each population plane.
+// D3Q125 mapping: one 128-lane block per cell, lane = target population (as the reference's
+// 125-thread block), the fixed 3x3x3 neighbourhood staged in LDS.
+#include "pond_device.h"
+
+namespace pond {
+
+// ---------------------------------------------------------------------------------------------
+// Gauge transfer, D3Q27 (structs.cuh:525-548)
+
+struct Tgt27 {
+  double u1x, u1y, u1z, T1;
+  double W;           // (wk*wl)*wm
+  double Cx, Cy, Cz;  // (e_inew*sqrt(|T1|))*sign(T1)
+  bool kx, ky, kz;    // e_inew component != 0
+};
+
+__device__ __forceinline__ Tgt27 make_tgt27(int inew, double u1x, double u1y, double u1z, double T1) {
+  Tgt27 t;
+  t.u1x = u1x; t.u1y = u1y; t.u1z = u1z; t.T1 = T1;
+  const int ex = kE27[inew][0], ey = kE27[inew][1], ez = kE27[inew][2];
+  double wk = 1. / (2 * T1), wl = wk, wm = wk;
+  if (ex == 0) wk = -1. / T1;
+  if (ey == 0) wl = -1. / T1;
+  if (ez == 0) wm = -1. / T1;
+  t.W = wk * wl * wm;
+  const double sq1 = sqrt(fabs(T1));
+  const double sg1 = (double)isign_d(T1);
+  t.Cx = (double)ex * sq1 * sg1;
+  t.Cy = (double)ey * sq1 * sg1;
+  t.Cz = (double)ez * sq1 * sg1;
+  t.kx = ex != 0; t.ky = ey != 0; t.kz = ez != 0;
+  return t;
+}
+
+// f: population planes at the node's cell index (stride `plane`); m: moment planes likewise
+__device__ __forceinline__ double transfer27(const double* __restrict__ f, const double* __restrict__ m,
+                                             int64_t plane, const Tgt27& t, double sqTb) {
+  const double u0x = m[1 * plane], u0y = m[2 * plane], u0z = m[3 * plane], T0 = m[4 * plane];
+  const double Dx = (t.u1x - u0x) * sqTb;
+  const double Dy = (t.u1y - u0y) * sqTb;
+  const double Dz = (t.u1z - u0z) * sqTb;
+  const double sq0 = sqrt(fabs(T0));
+  const double sg0 = (double)isign_d(T0);
+  // e_iold*sqrt(|T0|)*sign(T0) for e_iold = -1, 0, +1
+  const double E[3] = {-1.0 * sq0 * sg0, 0.0 * sq0 * sg0, 1.0 * sq0 * sg0};
+  double gx[3], gy[3], gz[3];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    const double ax = Dx - E[j], ay = Dy - E[j], az = Dz - E[j];
+    gx[j] = ax * ax - (t.kx ? t.Cx * ax : t.T1);
+    gy[j] = ay * ay - (t.ky ? t.Cy * ay : t.T1);
+    gz[j] = az * az - (t.kz ? t.Cz * az : t.T1);
+  }
+  double p1[3], p2[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    p1[a] = t.W * gx[a];
+#pragma unroll
+    for (int b = 0; b < 3; b++) p2[a][b] = p1[a] * gy[b];
+  }
+  double fnew = 0;
+#pragma unroll
+  for (int iold = 0; iold < 27; iold++) {
+    const int jx = kE27[iold][0] + 1, jy = kE27[iold][1] + 1, jz = kE27[iold][2] + 1;
+    const double fadd = p2[jx][jy] * gz[jz] * f[(int64_t)iold * plane];
+    if (finite_d(fadd)) fnew += fadd;
+  }
+  return fnew;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Gauge transfer, D1Q5 (structs.cuh:352-395)
+
+struct Tgt5 {
+  double u1x, et, wk, fnew0;
+  double e1sq, e2sq, E, etet;
+  bool k0;  // inew == 0
+};
+
+__device__ __forceinline__ Tgt5 make_tgt5(int inew, double u1x, double T1) {
+  Tgt5 t;
+  t.u1x = u1x;
+  const double s1 = sqrts_d(T1 / 1.0);
+  const double e1_to = kA5[1] * s1, e2_to = kA5[3] * s1;
+  t.et = kA5[inew] * s1;
+  const double ef = kA5[inew];
+  double wk;
+  if (ef == 0) wk = 1. / (e1_to * e1_to * e2_to * e2_to);
+  else if (fabs(ef) == kEc1) wk = 1. / (2 * e1_to * e1_to * (e1_to * e1_to - e2_to * e2_to));
+  else wk = 1. / (2 * e2_to * e2_to * (e2_to * e2_to - e1_to * e1_to));
+  t.wk = wk;
+  t.fnew0 = wk * 0.0;
+  t.e1sq = e1_to * e1_to;
+  t.e2sq = e2_to * e2_to;
+  t.E = t.e1sq + t.e2sq;
+  t.etet = t.et * t.et;
+  t.k0 = inew == 0;
+  return t;
+}
+
+__device__ __forceinline__ double transfer5(const double* __restrict__ f, const double* __restrict__ m,
+                                            int64_t plane, const Tgt5& t) {
+  const double u0x = m[1 * plane], T0 = m[4 * plane];
+  const double s0 = sqrts_d(T0 / 1.0);
+  const double et = t.et;
+  const double difu = t.u1x - u0x;
+  double fnew = t.fnew0;
+#pragma unroll
+  for (int iold = 0; iold < 5; iold++) {
+    const double e = kA5[iold] * s0;
+    double K0, K1, K2, K3;
+    const double K4 = 1;
+    K3 = -4 * e - et;
+    if (t.k0) {
+      K2 = 6 * e * e - t.E;
+      K1 = 2 * e * (t.E - 2 * e * e);
+      K0 = (t.e1sq - e * e) * (t.e2sq - e * e);
+    } else {
+      K2 = 6 * e * e + 3 * e * et - (t.E - t.etet);
+      K1 = (2 * e + et) * (t.E - t.etet) - e * e * (4 * e + 3 * et);
+      K0 = e * (e * e - t.E + t.etet) * (e + et);
+    }
+    double dm4 = u0x + e;
+    dm4 = dm4 * dm4 * dm4 * dm4;
+    dm4 *= 0;
+    const double gx = K4 * difu * difu * difu * difu + K3 * difu * difu * difu + K2 * difu * difu + K1 * difu + K0 - dm4;
+    const double fadd = t.wk * gx * f[(int64_t)iold * plane];
+    if (finite_d(fadd)) fnew += fadd;
+  }
+  return fnew;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Lattice traits for the one-lane-per-cell kernel
+
+template <int LAT>
+struct Lat;
+
+template <>
+struct Lat<kD3Q27> {
+  static constexpr int Q = 27;
+  using Tgt = Tgt27;
+  __device__ static double efc(int i, int d) { return (double)kE27[i][d]; }
+  __device__ static Tgt tgt(int i, const double* g) { return make_tgt27(i, g[0], g[1], g[2], g[3]); }
+  __device__ static double xfer(const double* f, const double* m, int64_t plane, const Tgt& t, double sqTb) {
+    return transfer27(f, m, plane, t, sqTb);
+  }
+};
+
+template <>
+struct Lat<kD1Q5> {
+  static constexpr int Q = 5;
+  using Tgt = Tgt5;
+  __device__ static double efc(int i, int d) { return d == 0 ? kA5[i] : 0.0; }
+  __device__ static Tgt tgt(int i, const double* g) { return make_tgt5(i, g[0], g[3]); }
+  __device__ static double xfer(const double* f, const double* m, int64_t plane, const Tgt& t, double) {
+    return transfer5(f, m, plane, t);
+  }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Interpolators
+
+// the two factors (s-p)/(i-p), p != i, p ascending, of the 3-node Lagrange basis (steps.cu:597)
+__device__ __forceinline__ void lag3_factors(double s, double F[3][2]) {
+  F[0][0] = (s - 1) / (0 - 1); F[0][1] = (s - 2) / (0 - 2);
+  F[1][0] = (s - 0) / (1 - 0); F[1][1] = (s - 2) / (1 - 2);
+  F[2][0] = (s - 0) / (2 - 0); F[2][1] = (s - 1) / (2 - 1);
+}
+
+__device__ __forceinline__ void lag4_factors(double s, double F[4][3]) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    int c = 0;
+#pragma unroll
+    for (int p = 0; p < 4; p++)
+      if (p != i) F[i][c++] = (s - p) / (double)(i - p);
+  }
+}
+
+struct NodeSrc {
+  const double* __restrict__ f;
+  const double* __restrict__ m;
+  Grid g;
+};
+
+// propagateWENO<3,5> (steps.cu:774-825) with LagrEval<3> (steps.cu:924-937).
+// CENTRED=false: as written, y/z sub-stencil nodes floor(y+0.5)+0..2 and smoothness samples on
+// the row (r mod Ny, r mod Nz), r = floor(x+0.5) (steps.cu:776-783).  This extrapolates
+// downwind in y and z and is unstable in 3D (grid-scale modes grow every step).
+// CENTRED=true: the corrected variant (not in the reference): y/z nodes floor(y+0.5)-1..+1 and
+// smoothness samples on the departure point's own row, which is the middle node of every
+// slice, so they reuse that slice's transfers.
+template <int LAT, bool CENTRED>
+__device__ double interp_weno5(const NodeSrc& src, double cx, double cy, double cz, const typename Lat<LAT>::Tgt& t,
+                               double sqTb) {
+  const Grid& g = src.g;
+  const int r = (int)floor(cx + 0.5);
+  const int ry = (int)floor(cy + 0.5) - (CENTRED ? 1 : 0), rz = (int)floor(cz + 0.5) - (CENTRED ? 1 : 0);
+  const double sy = cy - ry, sz = cz - rz;
+  double Fy[3][2], Fz[3][2];
+  lag3_factors(sy, Fy);
+  lag3_factors(sz, Fz);
+  // x parts of LagrPol<3> for sub-stencils k = 0,1,2 with left node r-2+k
+  double px[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    double Fx[3][2];
+    lag3_factors(cx - (r - 2 + k), Fx);
+#pragma unroll
+    for (int ix = 0; ix < 3; ix++) px[k][ix] = 1 * Fx[ix][0] * Fx[ix][1];
+  }
+  int yy[3], zz[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) { yy[a] = wrapi(ry + a, g.ny); zz[a] = wrapi(rz + a, g.nz); }
+  const bool line = (g.ny == 1) && (g.nz == 1);
+  const int by = CENTRED ? yy[1] : wrapi(r, g.ny), bz = CENTRED ? zz[1] : wrapi(r, g.nz);
+  double S0 = 0, S1 = 0, S2 = 0;
+  double ub[5];
+  for (int xs = 0; xs < 5; xs++) {
+    const int x = wrapi(r - 2 + xs, g.nx);
+    double T[9];
+    if (line) {
+      const int64_t c = x;
+      const double v = Lat<LAT>::xfer(src.f + c, src.m + c, g.plane, t, sqTb);
+#pragma unroll
+      for (int q = 0; q < 9; q++) T[q] = v;
+      ub[xs] = v;  // smoothness row (r mod 1, r mod 1) = (0,0): same node
+    } else {
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+          const int64_t c = cell_index(g, x, yy[a], zz[b]);
+          T[3 * a + b] = Lat<LAT>::xfer(src.f + c, src.m + c, g.plane, t, sqTb);
+        }
+      if (CENTRED) {
+        ub[xs] = T[4];
+      } else {
+        const int64_t cb = cell_index(g, x, by, bz);
+        ub[xs] = Lat<LAT>::xfer(src.f + cb, src.m + cb, g.plane, t, sqTb);
+      }
+    }
+    // accumulate into every sub-stencil that contains this x slice, in (iy, iz) order
+    if (xs <= 2) {
+      const double p = xs == 0 ? px[0][0] : xs == 1 ? px[0][1] : px[0][2];
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) S0 += p * Fy[a][0] * Fy[a][1] * Fz[b][0] * Fz[b][1] * T[3 * a + b];
+    }
+    if (xs >= 1 && xs <= 3) {
+      const double p = xs == 1 ? px[1][0] : xs == 2 ? px[1][1] : px[1][2];
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) S1 += p * Fy[a][0] * Fy[a][1] * Fz[b][0] * Fz[b][1] * T[3 * a + b];
+    }
+    if (xs >= 2) {
+      const double p = xs == 2 ? px[2][0] : xs == 3 ? px[2][1] : px[2][2];
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) S2 += p * Fy[a][0] * Fy[a][1] * Fz[b][0] * Fz[b][1] * T[3 * a + b];
+    }
+  }
+  const double cdist = cx - r;
+  const double d0 = (cdist * cdist - 3 * cdist + 2) / 12.;
+  const double d2 = (cdist * cdist + 3 * cdist + 2) / 12.;
+  const double d1 = 1. - d0 - d2;
+  const double u1 = ub[0], u2 = ub[1], u3 = ub[2], u4 = ub[3], u5 = ub[4];
+  const double beta0 = (10 * u3 * u3 + (11 * u1 - 31 * u2) * u3 + 25 * u2 * u2 - 19 * u1 * u2 + 4 * u1 * u1) / 3.0;
+  const double beta1 = (4 * u4 * u4 + (5 * u2 - 13 * u3) * u4 + 13 * u3 * u3 - 13 * u2 * u3 + 4 * u2 * u2) / 3.0;
+  const double beta2 = (4 * u5 * u5 + (11 * u3 - 19 * u4) * u5 + 25 * u4 * u4 - 31 * u3 * u4 + 10 * u3 * u3) / 3.0;
+  double w0 = d0 * 1, w1 = d1 * 1, w2 = d2 * 1;
+  const double sc = 1e-6;
+  w0 /= (sc + beta0) * (sc + beta0);
+  w1 /= (sc + beta1) * (sc + beta1);
+  w2 /= (sc + beta2) * (sc + beta2);
+  const double wsum = w0 + w1 + w2;
+  w0 /= wsum;
+  w1 /= wsum;
+  w2 /= wsum;
+  return w0 * S0 + w1 * S1 + w2 * S2;
+}
+
+// propagate<4> (steps.cu:986-1036): tricubic Lagrange on floor(x)-1 .. floor(x)+2
+template <int LAT>
+__device__ double interp_lagr4(const NodeSrc& src, double cx, double cy, double cz, const typename Lat<LAT>::Tgt& t,
+                               double sqTb) {
+  const Grid& g = src.g;
+  const int lx = (int)(floor(cx) - 4 / 2 + 1);
+  const int ly = (int)(floor(cy) - 4 / 2 + 1);
+  const int lz = (int)(floor(cz) - 4 / 2 + 1);
+  double Fx[4][3], Fy[4][3], Fz[4][3];
+  lag4_factors(cx - lx, Fx);
+  lag4_factors(cy - ly, Fy);
+  lag4_factors(cz - lz, Fz);
+  int yy[4], zz[4];
+#pragma unroll
+  for (int a = 0; a < 4; a++) { yy[a] = wrapi(ly + a, g.ny); zz[a] = wrapi(lz + a, g.nz); }
+  const bool line = (g.ny == 1) && (g.nz == 1);
+  double fpr = 0;
+  for (int ix = 0; ix < 4; ix++) {
+    const int x = wrapi(lx + ix, g.nx);
+    const double p = 1 * Fx[ix][0] * Fx[ix][1] * Fx[ix][2];
+    double v1 = 0;
+    if (line) v1 = Lat<LAT>::xfer(src.f + x, src.m + x, g.plane, t, sqTb);
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        double T;
+        if (line) {
+          T = v1;
+        } else {
+          const int64_t c = cell_index(g, x, yy[a], zz[b]);
+          T = Lat<LAT>::xfer(src.f + c, src.m + c, g.plane, t, sqTb);
+        }
+        fpr += p * Fy[a][0] * Fy[a][1] * Fy[a][2] * Fz[b][0] * Fz[b][1] * Fz[b][2] * T;
+      }
+  }
+  return fpr;
+}
+
+// ---------------------------------------------------------------------------------------------
+// One lane per cell: LBMpull (steps.cu:356-365, 421-512, 557-564, 582)
+
+template <int LAT, int INTERP>
+__global__ __launch_bounds__(64) void k_step_f64(const double* __restrict__ fin, const double* __restrict__ min,
+                                                 double* __restrict__ fout, double* __restrict__ mout,
+                                                 int* __restrict__ niter_out,
+                                                 const StepArgs a) {
+  constexpr int Q = Lat<LAT>::Q;
+  const Grid& g = a.g;
+  const int64_t cell = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  const bool active = cell < g.n;
+  int niter = 0;
+  if (active) {
+    const int x = (int)(cell % g.nx);
+    const int y = (int)((cell / g.nx) % g.ny);
+    const int z = (int)(cell / ((int64_t)g.nx * g.ny));
+    const int64_t P = g.plane;
+    // c0 = load; all-zero cell -> set_base(1, (0,0,0,TLat)) (steps.cu:360-363)
+    double rho0 = min[cell], c0[4] = {min[P + cell], min[2 * P + cell], min[3 * P + cell], min[4 * P + cell]};
+    const bool zero = rho0 == 0 && c0[0] == 0 && c0[1] == 0 && c0[2] == 0 && c0[3] == 0;
+    if (zero) { rho0 = 1; c0[3] = a.tlat; }
+    const NodeSrc src{fin, min, g};
+    while (niter < a.max_iter) {
+      niter++;
+      const double uT[4] = {c0[0], c0[1], c0[2], c0[3]};
+      const double s = sqrt(uT[3] / a.tlat);
+      double rho = 0, m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+      bool conv = true;
+      for (int i = 0; i < Q; i++) {
+        const double vx = s * Lat<LAT>::efc(i, 0) + uT[0];
+        const double vy = s * Lat<LAT>::efc(i, 1) + uT[1];
+        const double vz = s * Lat<LAT>::efc(i, 2) + uT[2];
+        const typename Lat<LAT>::Tgt t = Lat<LAT>::tgt(i, uT);
+        const double cx = (double)x - vx, cy = (double)y - vy, cz = (double)z - vz;
+        const double fn = INTERP == 1   ? interp_lagr4<LAT>(src, cx, cy, cz, t, a.sqrt_tbase)
+                          : INTERP == 2 ? interp_weno5<LAT, true>(src, cx, cy, cz, t, a.sqrt_tbase)
+                                        : interp_weno5<LAT, false>(src, cx, cy, cz, t, a.sqrt_tbase);
+        rho += fn;
+        m0 += fn * vx;
+        m1 += fn * vy;
+        m2 += fn * vz;
+        m3 += fn * (vx * vx + vy * vy + vz * vz);
+        const int64_t o = (int64_t)i * P + cell;
+        const double fo = niter > 1 ? fout[o] : (zero ? a.w[i] * 1.0 : fin[o]);
+        if (fabs(fn - fo) >= a.tol_abs + a.tol_rel * fabs(fn)) conv = false;
+        fout[o] = fn;
+      }
+      // Cell::calcMoments (structs.cuh:329-330)
+      if (rho != 0) { m0 /= rho; m1 /= rho; m2 /= rho; m3 /= rho; }
+      m3 = (m3 - m0 * m0 - m1 * m1 - m2 * m2) * a.inv_dim;
+      if (m3 < 0) m3 = c0[3];  // steps.cu:475-478
+      const double mn[4] = {m0, m1, m2, m3};
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if (fabs(mn[k] - c0[k]) >= a.tol_abs + a.tol_rel * fabs(mn[k])) conv = false;
+      rho0 = rho;
+      c0[0] = m0; c0[1] = m1; c0[2] = m2; c0[3] = m3;
+      if (conv) break;
+    }
+    // BGK towards rho*w_i in the converged gauge (steps.cu:557-564)
+    for (int i = 0; i < Q; i++) {
+      const int64_t o = (int64_t)i * P + cell;
+      double f = fout[o];
+      const double feq = rho0 * a.w[i];
+      f += a.dtau * (feq - f);
+      fout[o] = f;
+    }
+    mout[cell] = rho0;
+    mout[P + cell] = c0[0];
+    mout[2 * P + cell] = c0[1];
+    mout[3 * P + cell] = c0[2];
+    mout[4 * P + cell] = c0[3];
+    niter_out[cell] = niter;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// D3Q125: one block per cell, lane = target population (PonDstepD3Q125, steps.cu:151-250)
+
+__global__ __launch_bounds__(128) void k_step_q125_f64(const double* __restrict__ fin, const double* __restrict__ min,
+                                                       double* __restrict__ fout, double* __restrict__ mout,
+                                                       int* __restrict__ niter_out,
+                                                       const StepArgs a) {
+  constexpr int Q = 125;
+  __shared__ double sf[27][Q];   // fixed neighbourhood populations (cfix, steps.cu:150)
+  __shared__ double sg[27][4];   // their gauges
+  __shared__ double snew[Q];     // cnew.f
+  __shared__ int sflag[2];
+  const Grid& g = a.g;
+  const int64_t P = g.plane;
+  const int64_t cell = blockIdx.x;
+  const int k = threadIdx.x;
+  const bool lane_ok = k < Q;
+  const int x = (int)(cell % g.nx);
+  const int y = (int)((cell / g.nx) % g.ny);
+  const int z = (int)(cell / ((int64_t)g.nx * g.ny));
+  for (int n = 0; n < 27; n++) {
+    const int px = wrapi(x - 1 + n / 9, g.nx), py = wrapi(y - 1 + (n / 3) % 3, g.ny), pz = wrapi(z - 1 + n % 3, g.nz);
+    const int64_t c = cell_index(g, px, py, pz);
+    if (lane_ok) sf[n][k] = fin[(int64_t)k * P + c];
+    if (k < 4) sg[n][k] = min[(int64_t)(k + 1) * P + c];
+  }
+  double rho0 = min[cell], c0[4] = {min[P + cell], min[2 * P + cell], min[3 * P + cell], min[4 * P + cell]};
+  const bool zero = rho0 == 0 && c0[0] == 0 && c0[1] == 0 && c0[2] == 0 && c0[3] == 0;
+  if (zero) { rho0 = 1; c0[3] = a.tlat; }
+  double f0 = lane_ok ? (zero ? a.w[k] * 1.0 : fin[(int64_t)k * P + cell]) : 0.0;
+  __syncthreads();
+
+  // per-axis class of target k: index into {0,+ec1,-ec1,+ec2,-ec2}
+  const int kk = lane_ok ? k : 0;
+  const int X[3] = {kk / 25, (kk / 5) % 5, kk % 5};
+  int niter = 0;
+  while (niter < a.max_iter) {
+    niter++;
+    const double uT[4] = {c0[0], c0[1], c0[2], c0[3]};
+    const double s = sqrt(uT[3] / a.tlat);
+    // target constants (structs.cuh:397-416)
+    const double S1 = sqrts_d(uT[3] / 1.0);
+    const double e1_to = kEc1 * S1, e2_to = kEc2 * S1;
+    double wv[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const double en = kA5[X[d]];
+      if (en == 0) wv[d] = 1. / (e1_to * e1_to * e2_to * e2_to);
+      else if (fabs(en) == kEc1) wv[d] = 1. / (2 * e1_to * e1_to * (e1_to * e1_to - e2_to * e2_to));
+      else wv[d] = 1. / (2 * e2_to * e2_to * (e2_to * e2_to - e1_to * e1_to));
+    }
+    const double W = wv[0] * wv[1] * wv[2];
+    const double vk[3] = {s * kA5[X[0]] + uT[0], s * kA5[X[1]] + uT[1], s * kA5[X[2]] + uT[2]};
+    const double sh[3] = {((double)x - vk[0]) - (double)(x - 1), ((double)y - vk[1]) - (double)(y - 1),
+                          ((double)z - vk[2]) - (double)(z - 1)};
+    double Fx[3][2], Fy[3][2], Fz[3][2];
+    lag3_factors(sh[0], Fx);
+    lag3_factors(sh[1], Fy);
+    lag3_factors(sh[2], Fz);
+    // per-lane offsets of the four factors of the target's basis polynomial, in the cyclic
+    // order (X+1..X+4)%5 of structs.cuh:432-434: factor m is b + off[(X+m)%5] with
+    // off = {-0.0, -e1_to, +e1_to, -e2_to, +e2_to} (b - c == b + (-c) and b + (-0.0) == b exactly)
+    double off[3][4];
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+      for (int mm = 1; mm <= 4; mm++) {
+        const int c = (X[d] + mm) % 5;
+        off[d][mm - 1] = c == 0 ? -0.0 : c == 1 ? -e1_to : c == 2 ? e1_to : c == 3 ? -e2_to : e2_to;
+      }
+    double fpr = 0;
+    for (int n = 0; n < 27; n++) {
+      const double S0 = sqrts_d(sg[n][3] / 1.0);
+      double gb[3][5];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        const double du = sg[n][d] - uT[d];
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+          const double b = du + kA5[j] * S0;
+          gb[d][j] = (b + off[d][0]) * (b + off[d][1]) * (b + off[d][2]) * (b + off[d][3]);
+        }
+      }
+      double fnew = 0;
+#pragma unroll
+      for (int jx = 0; jx < 5; jx++)
+#pragma unroll
+        for (int jy = 0; jy < 5; jy++) {
+          const double gxy = gb[0][jx] * gb[1][jy];
+#pragma unroll
+          for (int jz = 0; jz < 5; jz++) {
+            const double fadd = W * (gxy * gb[2][jz]) * sf[n][25 * jx + 5 * jy + jz];
+            if (finite_d(fadd)) fnew += fadd;
+          }
+        }
+      const int ia = n / 9, ib = (n / 3) % 3, ic = n % 3;
+      fpr += 1 * Fx[ia][0] * Fx[ia][1] * Fy[ib][0] * Fy[ib][1] * Fz[ic][0] * Fz[ic][1] * fnew;
+    }
+    if (lane_ok) snew[k] = fpr;
+    __syncthreads();
+    // Cell::calcMoments on the shared cnew, serial in population order (every lane, same result)
+    double rho = 0, m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+    for (int i = 0; i < Q; i++) {
+      const double fi = snew[i];
+      const double vx = s * kA5[i / 25] + uT[0];
+      const double vy = s * kA5[(i / 5) % 5] + uT[1];
+      const double vz = s * kA5[i % 5] + uT[2];
+      rho += fi;
+      m0 += fi * vx;
+      m1 += fi * vy;
+      m2 += fi * vz;
+      m3 += fi * (vx * vx + vy * vy + vz * vz);
+    }
+    if (rho != 0) { m0 /= rho; m1 /= rho; m2 /= rho; m3 /= rho; }
+    m3 = (m3 - m0 * m0 - m1 * m1 - m2 * m2) * a.inv_dim;
+    if (m3 < 0) m3 = c0[3];
+    const double mn[4] = {m0, m1, m2, m3};
+    bool conv = true;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (fabs(mn[q] - c0[q]) >= a.tol_abs + a.tol_rel * fabs(mn[q])) conv = false;
+    if (lane_ok && fabs(fpr - f0) >= a.tol_abs + a.tol_rel * fabs(fpr)) conv = false;
+    const bool wave_conv = __all(conv);
+    if ((threadIdx.x & 63) == 0) sflag[threadIdx.x >> 6] = wave_conv ? 1 : 0;
+    __syncthreads();
+    const bool all_conv = sflag[0] && sflag[1];
+    rho0 = rho;
+    c0[0] = m0; c0[1] = m1; c0[2] = m2; c0[3] = m3;
+    f0 = fpr;
+    __syncthreads();  // sflag and snew are rewritten next iteration
+    if (all_conv) break;
+  }
+  if (lane_ok) {
+    const double feq = rho0 * a.w[k];
+    f0 += a.dtau * (feq - f0);
+    fout[(int64_t)k * P + cell] = f0;
+  }
+  if (k == 0) {
+    mout[cell] = rho0;
+    mout[P + cell] = c0[0];
+    mout[2 * P + cell] = c0[1];
+    mout[3 * P + cell] = c0[2];
+    mout[4 * P + cell] = c0[3];
+    niter_out[cell] = niter;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// host-side launchers
+
+hipError_t launch_step_f64(int lattice, int interp, const double* fin, const double* min, double* fout, double* mout,
+                           int* niter, const StepArgs& a, hipStream_t st) {
+  if (lattice == kD3Q125) {
+    const int64_t blocks = a.g.n;
+    hipLaunchKernelGGL(k_step_q125_f64, dim3((unsigned)blocks), dim3(128), 0, st, fin, min, fout, mout, niter, a);
+    return hipGetLastError();
+  }
+  const unsigned blocks = (unsigned)((a.g.n + 63) / 64);
+#define POND_LAUNCH(L, I)                                                                                  \
+  hipLaunchKernelGGL((k_step_f64<L, I>), dim3(blocks), dim3(64), 0, st, fin, min, fout, mout, niter, a)
+  if (lattice == kD3Q27) {
+    if (interp == 0) POND_LAUNCH(kD3Q27, 0);
+    else if (interp == 1) POND_LAUNCH(kD3Q27, 1);
+    else POND_LAUNCH(kD3Q27, 2);
+  } else {
+    // on a line (Ny = Nz = 1) the centred and as-written WENO5 coincide
+    if (interp == 1) POND_LAUNCH(kD1Q5, 1);
+    else POND_LAUNCH(kD1Q5, 0);
+  }
+#undef POND_LAUNCH
+  return hipGetLastError();
+}
+
+}  // namespace pond
