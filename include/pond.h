@@ -138,6 +138,41 @@ pond_status pond_drop(pond_ctx* ctx, const char* dir);
    (nx*ny*nz doubles); dims[3] receives Nx,Ny,Nz.  out may be NULL to query dims. */
 pond_status pond_read_arr(const char* path, int64_t* dims, double* out);
 
+/* ---- learned gauge initializer (no counterpart in the reference) ----
+   A 135-64-64-4 ReLU network in BF16 (FP32 accumulate) predicts each cell's converged gauge of
+   the next step from the previous step's (rho,u,T) on its 3x3x3 neighbourhood.  It is trained
+   on the GPU from samples the solver itself produces.  A POND_SPLIT context created with
+   use_mlp=1 uses it for the starting gauge once it has been trained or set. */
+
+/* number of FP32 parameters, packed W1[135][64] b1[64] W2[64][64] b2[64] W3[64][4] b3[4] */
+int64_t pond_mlp_num_params(void);
+
+/* seeded He-uniform initialisation (also resets the Adam state) */
+pond_status pond_mlp_init(pond_ctx* ctx, uint64_t seed);
+
+/* append nsamples seeded-random cells of the last step: features from the state before the
+   step, targets from the converged gauge after it (POND_SPLIT contexts) */
+pond_status pond_mlp_collect(pond_ctx* ctx, int64_t nsamples, uint64_t seed);
+pond_status pond_mlp_dataset_size(pond_ctx* ctx, int64_t* n);
+
+/* train on the collected samples: epochs of shuffled minibatches (batch a multiple of 16),
+   Adam with learning rate lr; epoch_loss[epochs] (may be NULL) receives each epoch's mean
+   normalized squared error over every 64th batch */
+pond_status pond_mlp_train(pond_ctx* ctx, int epochs, int batch, float lr, uint64_t seed, float* epoch_loss);
+
+/* parameters (kMlp params) and scales[9] = {1/scale of (rho,ux,uy,uz,T) features, output
+   scales of (dux,duy,duz,dT)} */
+pond_status pond_mlp_get_params(pond_ctx* ctx, float* params, float* scales);
+pond_status pond_mlp_set_params(pond_ctx* ctx, const float* params, const float* scales);
+
+/* network on a host batch of already-normalized features X[B][135] -> Y[B][4] (output
+   scales applied), through the fused MFMA inference kernel */
+pond_status pond_mlp_forward(pond_ctx* ctx, const float* X, int64_t B, float* Y);
+
+/* gradient of the mean-squared loss of a host batch (X[B][135], targets T[B][4], B a multiple
+   of 16) through the MFMA training kernels; grad has pond_mlp_num_params() entries */
+pond_status pond_mlp_grad(pond_ctx* ctx, const float* X, const float* T, int64_t B, float* grad, float* loss);
+
 /* device bytes held by the ctx (state buffers, scratch, network) */
 int64_t pond_device_bytes(const pond_ctx* ctx);
 

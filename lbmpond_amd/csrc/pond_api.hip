@@ -42,6 +42,17 @@ __global__ void k_set_base(FT* __restrict__ f, const double* __restrict__ rho, i
   for (int i = 0; i < qn; i++) f[(int64_t)i * plane + c] = (FT)(a.w[i] * r);
 }
 
+// split layout from FP64 moment planes: f_i = w_i*rho in FP32, (rho,u) in FP32, T in FP64
+__global__ void k_set_split(const double* __restrict__ m, float* __restrict__ f, float* __restrict__ mf,
+                            double* __restrict__ T, int64_t n, int64_t plane, int qn, const StepArgs a) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const double r = m[c];
+  for (int i = 0; i < qn; i++) f[(int64_t)i * plane + c] = (float)(a.w[i] * r);
+  for (int k = 0; k < 4; k++) mf[(int64_t)k * plane + c] = (float)m[(int64_t)k * plane + c];
+  T[c] = m[(int64_t)4 * plane + c];
+}
+
 // block-level deterministic partial sums: mass, momentum_x, energy, k, niter sum; niter max
 struct RedOut {
   double s[5];
@@ -96,6 +107,31 @@ __global__ __launch_bounds__(256) void k_reduce_f64(const double* __restrict__ m
   if (threadIdx.x == 0) out[blockIdx.x] = r;
 }
 
+// the same reductions for the split layout (FP32 rho,u and FP64 T)
+__global__ __launch_bounds__(256) void k_reduce_split(const float* __restrict__ m, const double* __restrict__ Tp,
+                                                      const int* __restrict__ niter, int64_t n, int64_t plane,
+                                                      double dim, double cx, double cy, double cz,
+                                                      RedOut* __restrict__ out) {
+  __shared__ double shd[4];
+  __shared__ int shi[4];
+  double s[5] = {0, 0, 0, 0, 0};
+  int imax = 0;
+  const double d8pi3 = 1.0 / (8 * M_PI * M_PI * M_PI);
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+    const double rho = m[c], ux = m[plane + c], uy = m[2 * plane + c], uz = m[3 * plane + c], T = Tp[c];
+    s[0] += rho;
+    s[1] += rho * ux;
+    s[2] += rho * (ux * ux + uy * uy + uz * uz + dim * T);
+    s[3] += 0.5 * d8pi3 * (ux * ux + uy * uy + uz * uz) * cx * cy * cz;
+    s[4] += (double)niter[c];
+    imax = max(imax, niter[c]);
+  }
+  RedOut r;
+  for (int q = 0; q < 5; q++) r.s[q] = block_sum(s[q], shd);
+  r.imax = block_max(imax, shi);
+  if (threadIdx.x == 0) out[blockIdx.x] = r;
+}
+
 __global__ void k_reduce_final(const RedOut* __restrict__ part, int nb, RedOut* __restrict__ out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   RedOut r{};
@@ -133,6 +169,12 @@ pond_status collect_events(pond_ctx* ctx) {
   }
   ctx->last_ms = ms_sum;
   ctx->total_ms += ms_sum;
+  if (ctx->mlp_pending) {
+    float mm = 0;
+    POND_HIP(ctx, hipEventElapsedTime(&mm, ctx->mlp_ev[0], ctx->mlp_ev[1]));
+    ctx->last_mlp_ms = mm;
+    ctx->mlp_pending = false;
+  }
   ctx->timed_steps += (int64_t)ctx->ev_pending.size();
   ctx->ev_pending.clear();
   return POND_OK;
@@ -192,7 +234,10 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
   if (p->interp != POND_WENO5 && p->interp != POND_LAGR4 && p->interp != POND_WENO5C) return bail(POND_ERR_INVALID, "pond_create: unknown interp");
   if (p->precision != POND_FP64 && p->precision != POND_SPLIT)
     return bail(POND_ERR_INVALID, "pond_create: unknown precision");
-  if (p->precision == POND_SPLIT) return bail(POND_ERR_INVALID, "pond_create: POND_SPLIT is not built in this library");
+  if (p->precision == POND_SPLIT && p->lattice != POND_D3Q27)
+    return bail(POND_ERR_INVALID, "pond_create: POND_SPLIT is built for D3Q27 only");
+  if (p->precision == POND_SPLIT && !(p->split_tol_rel >= 0 && p->split_tol_abs >= 0))
+    return bail(POND_ERR_INVALID, "pond_create: split tolerances must be >= 0");
   if (p->max_iter < 1) return bail(POND_ERR_INVALID, "pond_create: max_iter must be >= 1");
   if (p->use_mlp && p->precision != POND_SPLIT)
     return bail(POND_ERR_INVALID, "pond_create: the learned initializer requires POND_SPLIT");
@@ -214,12 +259,45 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
 
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(POND_ERR_HIP, "pond_create: hipStreamCreate failed");
-  const size_t fbytes = (size_t)ctx->Qn * g.plane * sizeof(double);
-  const size_t mbytes = (size_t)5 * g.plane * sizeof(double);
-  for (int b = 0; b < 2; b++) {
-    if (hipMalloc(&ctx->f64[b].f, fbytes) != hipSuccess || hipMalloc(&ctx->f64[b].m, mbytes) != hipSuccess)
-      return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of the state buffers failed");
-    ctx->bytes += (int64_t)(fbytes + mbytes);
+  ctx->split = p->precision == POND_SPLIT;
+  if (!ctx->split) {
+    const size_t fbytes = (size_t)ctx->Qn * g.plane * sizeof(double);
+    const size_t mbytes = (size_t)5 * g.plane * sizeof(double);
+    for (int b = 0; b < 2; b++) {
+      if (hipMalloc(&ctx->f64[b].f, fbytes) != hipSuccess || hipMalloc(&ctx->f64[b].m, mbytes) != hipSuccess)
+        return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of the state buffers failed");
+      ctx->bytes += (int64_t)(fbytes + mbytes);
+    }
+  } else {
+    const size_t fbytes = (size_t)ctx->Qn * g.plane * sizeof(float);
+    const size_t mbytes = (size_t)4 * g.plane * sizeof(float);
+    const size_t tbytes = (size_t)g.plane * sizeof(double);
+    for (int b = 0; b < 2; b++) {
+      if (hipMalloc(&ctx->sp[b].f, fbytes) != hipSuccess || hipMalloc(&ctx->sp[b].m, mbytes) != hipSuccess ||
+          hipMalloc(&ctx->sp[b].T, tbytes) != hipSuccess)
+        return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of the split state buffers failed");
+      ctx->bytes += (int64_t)(fbytes + mbytes + tbytes);
+    }
+    if (hipMalloc(&ctx->sq, (size_t)g.plane * sizeof(float)) != hipSuccess)
+      return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of the sqrt(T) plane failed");
+    ctx->bytes += (int64_t)g.plane * sizeof(float);
+    if (p->use_mlp) {
+      if (hipMalloc(&ctx->guess, (size_t)4 * g.plane * sizeof(float)) != hipSuccess)
+        return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of the predicted-gauge planes failed");
+      ctx->bytes += (int64_t)4 * g.plane * sizeof(float);
+      if (hipEventCreate(&ctx->mlp_ev[0]) != hipSuccess || hipEventCreate(&ctx->mlp_ev[1]) != hipSuccess)
+        return bail(POND_ERR_HIP, "pond_create: hipEventCreate failed");
+    }
+    SplitArgs& sa = ctx->sargs;
+    sa.g = g;
+    sa.dtau = p->dtau;
+    sa.tol_abs = p->split_tol_abs;
+    sa.tol_rel = p->split_tol_rel;
+    sa.max_iter = p->max_iter;
+    sa.sqrt_tbase = a.sqrt_tbase;
+    sa.tlat = a.tlat;
+    sa.inv_dim = a.inv_dim;
+    for (int i = 0; i < ctx->Qn; i++) sa.w[i] = (float)ctx->L.w[i];
   }
   if (hipMalloc(&ctx->niter, (size_t)g.plane * sizeof(int)) != hipSuccess)
     return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of niter failed");
@@ -244,7 +322,12 @@ void pond_destroy(pond_ctx* ctx) {
     if (ctx->sp[b].m) (void)hipFree(ctx->sp[b].m);
     if (ctx->sp[b].T) (void)hipFree(ctx->sp[b].T);
   }
+  pond_mlp_free(ctx);
+  for (auto& e : ctx->mlp_ev)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->niter) (void)hipFree(ctx->niter);
+  if (ctx->sq) (void)hipFree(ctx->sq);
+  if (ctx->guess) (void)hipFree(ctx->guess);
   if (ctx->red) (void)hipFree(ctx->red);
   for (auto& e : ctx->ev_pool) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
   for (auto& e : ctx->ev_pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
@@ -267,17 +350,38 @@ pond_status pond_set_state(pond_ctx* ctx, const double* rho, const double* ux, c
                            const double* T) {
   if (!ctx) return POND_ERR_INVALID;
   const Grid& g = ctx->g;
-  pond_status s = upload_moments(ctx, ctx->f64[0].m, rho, ux, uy, uz, T);
-  if (s != POND_OK) return s;
   const unsigned blocks = (unsigned)((g.n + 255) / 256);
-  hipLaunchKernelGGL(k_set_base<double>, dim3(blocks), dim3(256), 0, ctx->stream, ctx->f64[0].f, ctx->f64[0].m, g.n,
-                     g.plane, ctx->Qn, ctx->args);
-  POND_HIP(ctx, hipGetLastError());
-  // both buffers hold the initial state, as fill() does (init.cu:104-105)
-  POND_HIP(ctx, hipMemcpyAsync(ctx->f64[1].f, ctx->f64[0].f, (size_t)ctx->Qn * g.plane * sizeof(double),
-                               hipMemcpyDeviceToDevice, ctx->stream));
-  POND_HIP(ctx, hipMemcpyAsync(ctx->f64[1].m, ctx->f64[0].m, (size_t)5 * g.plane * sizeof(double),
-                               hipMemcpyDeviceToDevice, ctx->stream));
+  if (ctx->split) {
+    double* tmp = nullptr;
+    POND_HIP(ctx, hipMalloc(&tmp, (size_t)5 * g.plane * sizeof(double)));
+    pond_status s = upload_moments(ctx, tmp, rho, ux, uy, uz, T);
+    if (s == POND_OK) {
+      hipLaunchKernelGGL(k_set_split, dim3(blocks), dim3(256), 0, ctx->stream, tmp, ctx->sp[0].f, ctx->sp[0].m,
+                         ctx->sp[0].T, g.n, g.plane, ctx->Qn, ctx->args);
+      hipError_t e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) s = fail(ctx, POND_ERR_HIP, std::string("k_set_split: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(tmp);
+    if (s != POND_OK) return s;
+    POND_HIP(ctx, hipMemcpyAsync(ctx->sp[1].f, ctx->sp[0].f, (size_t)ctx->Qn * g.plane * sizeof(float),
+                                 hipMemcpyDeviceToDevice, ctx->stream));
+    POND_HIP(ctx, hipMemcpyAsync(ctx->sp[1].m, ctx->sp[0].m, (size_t)4 * g.plane * sizeof(float),
+                                 hipMemcpyDeviceToDevice, ctx->stream));
+    POND_HIP(ctx, hipMemcpyAsync(ctx->sp[1].T, ctx->sp[0].T, (size_t)g.plane * sizeof(double),
+                                 hipMemcpyDeviceToDevice, ctx->stream));
+  } else {
+    pond_status s = upload_moments(ctx, ctx->f64[0].m, rho, ux, uy, uz, T);
+    if (s != POND_OK) return s;
+    hipLaunchKernelGGL(k_set_base<double>, dim3(blocks), dim3(256), 0, ctx->stream, ctx->f64[0].f, ctx->f64[0].m,
+                       g.n, g.plane, ctx->Qn, ctx->args);
+    POND_HIP(ctx, hipGetLastError());
+    // both buffers hold the initial state, as fill() does (init.cu:104-105)
+    POND_HIP(ctx, hipMemcpyAsync(ctx->f64[1].f, ctx->f64[0].f, (size_t)ctx->Qn * g.plane * sizeof(double),
+                                 hipMemcpyDeviceToDevice, ctx->stream));
+    POND_HIP(ctx, hipMemcpyAsync(ctx->f64[1].m, ctx->f64[0].m, (size_t)5 * g.plane * sizeof(double),
+                                 hipMemcpyDeviceToDevice, ctx->stream));
+  }
   POND_HIP(ctx, hipMemsetAsync(ctx->niter, 0, (size_t)g.plane * sizeof(int), ctx->stream));
   POND_HIP(ctx, hipStreamSynchronize(ctx->stream));
   ctx->cur = 0;
@@ -291,12 +395,32 @@ pond_status pond_set_populations(pond_ctx* ctx, const double* f, const double* r
   if (!ctx) return POND_ERR_INVALID;
   if (!f) return fail(ctx, POND_ERR_INVALID, "pond_set_populations: null f");
   const Grid& g = ctx->g;
-  for (int b = 0; b < 2; b++) {
-    pond_status s = upload_moments(ctx, ctx->f64[b].m, rho, ux, uy, uz, T);
-    if (s != POND_OK) return s;
-    for (int i = 0; i < ctx->Qn; i++)
-      POND_HIP(ctx, hipMemcpyAsync(ctx->f64[b].f + (size_t)i * g.plane, f + (size_t)i * g.n,
-                                   (size_t)g.n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  if (ctx->split) {
+    const double* src[5] = {rho, ux, uy, uz, T};
+    for (int k = 0; k < 5; k++)
+      if (!src[k]) return fail(ctx, POND_ERR_INVALID, "null field pointer");
+    std::vector<float> tmp((size_t)g.n);
+    for (int b = 0; b < 2; b++) {
+      for (int i = 0; i < ctx->Qn; i++) {
+        for (int64_t c = 0; c < g.n; c++) tmp[c] = (float)f[(size_t)i * g.n + c];
+        POND_HIP(ctx, hipMemcpy(ctx->sp[b].f + (size_t)i * g.plane, tmp.data(), (size_t)g.n * sizeof(float),
+                                hipMemcpyHostToDevice));
+      }
+      for (int k = 0; k < 4; k++) {
+        for (int64_t c = 0; c < g.n; c++) tmp[c] = (float)src[k][c];
+        POND_HIP(ctx, hipMemcpy(ctx->sp[b].m + (size_t)k * g.plane, tmp.data(), (size_t)g.n * sizeof(float),
+                                hipMemcpyHostToDevice));
+      }
+      POND_HIP(ctx, hipMemcpy(ctx->sp[b].T, T, (size_t)g.n * sizeof(double), hipMemcpyHostToDevice));
+    }
+  } else {
+    for (int b = 0; b < 2; b++) {
+      pond_status s = upload_moments(ctx, ctx->f64[b].m, rho, ux, uy, uz, T);
+      if (s != POND_OK) return s;
+      for (int i = 0; i < ctx->Qn; i++)
+        POND_HIP(ctx, hipMemcpyAsync(ctx->f64[b].f + (size_t)i * g.plane, f + (size_t)i * g.n,
+                                     (size_t)g.n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    }
   }
   POND_HIP(ctx, hipMemsetAsync(ctx->niter, 0, (size_t)g.plane * sizeof(int), ctx->stream));
   POND_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -320,9 +444,31 @@ pond_status pond_step(pond_ctx* ctx, int nsteps) {
     EventPair e = ctx->ev_pool.back();
     ctx->ev_pool.pop_back();
     const int in = ctx->cur, outb = 1 - ctx->cur;
+    hipError_t he = hipSuccess;
+    if (ctx->split) {
+      he = launch_sqrt_t(ctx->sp[in].T, ctx->sq, ctx->g.n, ctx->stream);
+      if (he != hipSuccess) return fail(ctx, POND_ERR_HIP, std::string("sqrt(T) kernel: ") + hipGetErrorString(he));
+    }
     POND_HIP(ctx, hipEventRecord(e.a, ctx->stream));
-    hipError_t he = launch_step_f64(ctx->p.lattice, ctx->p.interp, ctx->f64[in].f, ctx->f64[in].m, ctx->f64[outb].f,
-                                    ctx->f64[outb].m, ctx->niter, ctx->args, ctx->stream);
+    const bool learned = ctx->split && ctx->p.use_mlp && ctx->mlp.trained;
+    if (learned) {
+      // predicted starting gauge from the load state (part of the step's timed work)
+      POND_HIP(ctx, hipEventRecord(ctx->mlp_ev[0], ctx->stream));
+      he = mlp_launch_infer(pond_mlp_weights(ctx), MlpState{ctx->g, ctx->sp[in].m, ctx->sp[in].T}, ctx->guess,
+                            ctx->g.plane, ctx->stream);
+      if (he != hipSuccess) return fail(ctx, POND_ERR_HIP, std::string("mlp inference: ") + hipGetErrorString(he));
+      POND_HIP(ctx, hipEventRecord(ctx->mlp_ev[1], ctx->stream));
+      ctx->mlp_pending = true;
+    }
+    if (ctx->split) {
+      const SplitPtrs pin{ctx->sp[in].f, ctx->sp[in].m, ctx->sp[in].T};
+      const SplitPtrs pout{ctx->sp[outb].f, ctx->sp[outb].m, ctx->sp[outb].T};
+      he = launch_step_split(ctx->p.lattice, ctx->p.interp, pin, pout, ctx->sq, ctx->niter,
+                             learned ? ctx->guess : nullptr, ctx->sargs, ctx->stream);
+    } else {
+      he = launch_step_f64(ctx->p.lattice, ctx->p.interp, ctx->f64[in].f, ctx->f64[in].m, ctx->f64[outb].f,
+                           ctx->f64[outb].m, ctx->niter, ctx->args, ctx->stream);
+    }
     if (he != hipSuccess) return fail(ctx, POND_ERR_HIP, std::string("step kernel launch: ") + hipGetErrorString(he));
     POND_HIP(ctx, hipEventRecord(e.b, ctx->stream));
     ctx->ev_pending.push_back(e);
@@ -345,10 +491,21 @@ pond_status pond_get_fields(pond_ctx* ctx, double* rho, double* ux, double* uy, 
   if (s != POND_OK) return s;
   double* dst[5] = {rho, ux, uy, uz, T};
   const Grid& g = ctx->g;
-  for (int k = 0; k < 5; k++)
-    if (dst[k])
-      POND_HIP(ctx, hipMemcpy(dst[k], ctx->f64[ctx->cur].m + (size_t)k * g.plane, (size_t)g.n * sizeof(double),
-                              hipMemcpyDeviceToHost));
+  if (ctx->split) {
+    std::vector<float> tmp((size_t)g.n);
+    for (int k = 0; k < 4; k++)
+      if (dst[k]) {
+        POND_HIP(ctx, hipMemcpy(tmp.data(), ctx->sp[ctx->cur].m + (size_t)k * g.plane, (size_t)g.n * sizeof(float),
+                                hipMemcpyDeviceToHost));
+        for (int64_t c = 0; c < g.n; c++) dst[k][c] = tmp[c];
+      }
+    if (T) POND_HIP(ctx, hipMemcpy(T, ctx->sp[ctx->cur].T, (size_t)g.n * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    for (int k = 0; k < 5; k++)
+      if (dst[k])
+        POND_HIP(ctx, hipMemcpy(dst[k], ctx->f64[ctx->cur].m + (size_t)k * g.plane, (size_t)g.n * sizeof(double),
+                                hipMemcpyDeviceToHost));
+  }
   if (niter) POND_HIP(ctx, hipMemcpy(niter, ctx->niter, (size_t)g.n * sizeof(int), hipMemcpyDeviceToHost));
   return POND_OK;
 }
@@ -360,6 +517,15 @@ pond_status pond_get_populations(pond_ctx* ctx, double* f) {
   pond_status s = pond_sync(ctx);
   if (s != POND_OK) return s;
   const Grid& g = ctx->g;
+  if (ctx->split) {
+    std::vector<float> tmp((size_t)g.n);
+    for (int i = 0; i < ctx->Qn; i++) {
+      POND_HIP(ctx, hipMemcpy(tmp.data(), ctx->sp[ctx->cur].f + (size_t)i * g.plane, (size_t)g.n * sizeof(float),
+                              hipMemcpyDeviceToHost));
+      for (int64_t c = 0; c < g.n; c++) f[(size_t)i * g.n + c] = tmp[c];
+    }
+    return POND_OK;
+  }
   for (int i = 0; i < ctx->Qn; i++)
     POND_HIP(ctx, hipMemcpy(f + (size_t)i * g.n, ctx->f64[ctx->cur].f + (size_t)i * g.plane,
                             (size_t)g.n * sizeof(double), hipMemcpyDeviceToHost));
@@ -372,8 +538,12 @@ pond_status pond_diagnostics(pond_ctx* ctx, pond_diag* out) {
   const Grid& g = ctx->g;
   auto* part = reinterpret_cast<RedOut*>(ctx->red);
   const int nb = (int)std::min<int64_t>(kRedBlocks, (g.n + 255) / 256);
-  hipLaunchKernelGGL(k_reduce_f64, dim3(nb), dim3(256), 0, ctx->stream, ctx->f64[ctx->cur].m, ctx->niter, g.n, g.plane,
-                     ctx->L.Dim, 2 * M_PI / g.nx, 2 * M_PI / g.ny, 2 * M_PI / g.nz, part);
+  if (ctx->split)
+    hipLaunchKernelGGL(k_reduce_split, dim3(nb), dim3(256), 0, ctx->stream, ctx->sp[ctx->cur].m, ctx->sp[ctx->cur].T,
+                       ctx->niter, g.n, g.plane, ctx->L.Dim, 2 * M_PI / g.nx, 2 * M_PI / g.ny, 2 * M_PI / g.nz, part);
+  else
+    hipLaunchKernelGGL(k_reduce_f64, dim3(nb), dim3(256), 0, ctx->stream, ctx->f64[ctx->cur].m, ctx->niter, g.n,
+                       g.plane, ctx->L.Dim, 2 * M_PI / g.nx, 2 * M_PI / g.ny, 2 * M_PI / g.nz, part);
   POND_HIP(ctx, hipGetLastError());
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, ctx->stream, part, nb, part + kRedBlocks);
   POND_HIP(ctx, hipGetLastError());

@@ -6,6 +6,8 @@
 #include "../../include/pond.h"
 #include "lattice.h"
 #include "pond_device.h"
+#include "step_split.h"
+#include "mlp.h"
 
 namespace pond {
 
@@ -27,6 +29,33 @@ struct EventPair {
   hipEvent_t a = nullptr, b = nullptr;
 };
 
+// learned gauge initializer: network, training state and dataset
+struct MlpCtx {
+  bool initialized = false, trained = false;
+  float* params = nullptr;  // FP32 master parameters (kMlpParams, packed)
+  float* grad = nullptr;
+  float* adam_m = nullptr;
+  float* adam_v = nullptr;
+  int64_t adam_t = 0;
+  unsigned short* W1b = nullptr;  // bf16 copies: W1 144x64, W2 64x64, W3 64x16
+  unsigned short* W2b = nullptr;
+  unsigned short* W3b = nullptr;
+  float* gw3tmp = nullptr;
+  double* mom = nullptr;
+  float in_inv[5] = {1, 1, 1, 1, 1};
+  float out_scale[4] = {1, 1, 1, 1};
+  float out_inv[4] = {1, 1, 1, 1};
+  float* X = nullptr;  // dataset: raw features (cap x 135) and targets (cap x 4)
+  float* Y = nullptr;
+  int64_t count = 0, cap = 0;
+  int bcap = 0;  // minibatch buffers
+  float *bx = nullptr, *bt = nullptr, *z1acc = nullptr, *z1 = nullptr, *z2acc = nullptr, *z2 = nullptr,
+        *yacc = nullptr, *y = nullptr, *dy = nullptr, *g2 = nullptr, *dz2 = nullptr, *g1 = nullptr, *dz1 = nullptr,
+        *sqerr = nullptr, *loss = nullptr;
+  unsigned short *xb = nullptr, *a1b = nullptr, *a2b = nullptr, *dyb = nullptr, *dz2b = nullptr, *dz1b = nullptr;
+  int* idx = nullptr;
+};
+
 hipError_t launch_step_f64(int lattice, int interp, const double* fin, const double* min, double* fout,
                            double* mout, int* niter, const StepArgs& a, hipStream_t st);
 
@@ -41,8 +70,12 @@ struct pond_ctx {
   hipStream_t stream = nullptr;
   int cur = 0;              // index of the buffer holding the current state
   bool has_state = false;
+  bool split = false;       // POND_SPLIT state layout
   pond::F64Buf f64[2];
   pond::SplitBuf sp[2];
+  pond::SplitArgs sargs{};
+  float* sq = nullptr;      // split path: sqrt(|T|)*sign(T) per cell of the load buffer
+  float* guess = nullptr;   // split path: learned starting gauge increments (4 planes)
   int* niter = nullptr;     // per-cell gauge iterations of the last step
   double* red = nullptr;    // reduction partials
   int64_t red_len = 0;
@@ -53,5 +86,11 @@ struct pond_ctx {
   std::vector<pond::EventPair> ev_pending;
   double last_ms = 0, total_ms = 0, last_mlp_ms = 0;
   int64_t timed_steps = 0;
+  pond::MlpCtx mlp;
+  hipEvent_t mlp_ev[2] = {nullptr, nullptr};  // around the last inference launch
+  bool mlp_pending = false;
   std::string err;
 };
+
+pond::MlpWeightsBF pond_mlp_weights(const pond_ctx* ctx);
+void pond_mlp_free(pond_ctx* ctx);

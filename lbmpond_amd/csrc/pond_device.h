@@ -47,6 +47,19 @@ __device__ __forceinline__ int wrapi(int v, int n) {
   return r < 0 ? r + n : r;
 }
 
+// floor(v) as a stencil coordinate.  A double->int conversion of NaN or of a value outside the
+// int range is undefined (LLVM: poison), and a poisoned coordinate turns into an arbitrary
+// address after the periodic wrap.  NaN maps to 0, as CUDA's conversion does, and values are
+// clamped to +-2^30 cells so that small offsets added to the result cannot overflow.  For every
+// finite departure point closer than 2^30 cells the result is exactly (int)floor(v).
+__device__ __forceinline__ int floor_int(double v) {
+  const double f = floor(v);
+  if (!(f == f)) return 0;
+  if (f > 1073741824.0) return 1073741824;
+  if (f < -1073741824.0) return -1073741824;
+  return (int)f;
+}
+
 __device__ __forceinline__ int64_t cell_index(const Grid& g, int x, int y, int z) {
   return (int64_t)x + (int64_t)g.nx * ((int64_t)y + (int64_t)g.ny * (int64_t)z);
 }

@@ -212,8 +212,8 @@ template <int LAT, bool CENTRED>
 __device__ double interp_weno5(const NodeSrc& src, double cx, double cy, double cz, const typename Lat<LAT>::Tgt& t,
                                double sqTb) {
   const Grid& g = src.g;
-  const int r = (int)floor(cx + 0.5);
-  const int ry = (int)floor(cy + 0.5) - (CENTRED ? 1 : 0), rz = (int)floor(cz + 0.5) - (CENTRED ? 1 : 0);
+  const int r = floor_int(cx + 0.5);
+  const int ry = floor_int(cy + 0.5) - (CENTRED ? 1 : 0), rz = floor_int(cz + 0.5) - (CENTRED ? 1 : 0);
   const double sy = cy - ry, sz = cz - rz;
   double Fy[3][2], Fz[3][2];
   lag3_factors(sy, Fy);
@@ -306,9 +306,9 @@ template <int LAT>
 __device__ double interp_lagr4(const NodeSrc& src, double cx, double cy, double cz, const typename Lat<LAT>::Tgt& t,
                                double sqTb) {
   const Grid& g = src.g;
-  const int lx = (int)(floor(cx) - 4 / 2 + 1);
-  const int ly = (int)(floor(cy) - 4 / 2 + 1);
-  const int lz = (int)(floor(cz) - 4 / 2 + 1);
+  const int lx = floor_int(cx) - 4 / 2 + 1;
+  const int ly = floor_int(cy) - 4 / 2 + 1;
+  const int lz = floor_int(cz) - 4 / 2 + 1;
   double Fx[4][3], Fy[4][3], Fz[4][3];
   lag4_factors(cx - lx, Fx);
   lag4_factors(cy - ly, Fy);

@@ -1,0 +1,29 @@
+// step_split.h -- arguments of the FP32/FP64-split step kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "pond_device.h"
+
+namespace pond {
+
+struct SplitArgs {
+  Grid g;
+  double dtau;
+  double tol_abs, tol_rel;  // FP64 gauge residual tolerances (thermal-speed / T scaled)
+  int max_iter;
+  double sqrt_tbase;
+  double tlat;
+  double inv_dim;
+  float w[kQmax];
+};
+
+struct SplitPtrs {
+  float* f;   // Qn planes
+  float* m;   // rho, ux, uy, uz planes
+  double* T;  // temperature plane
+};
+
+hipError_t launch_sqrt_t(const double* T, float* sq, int64_t n, hipStream_t st);
+hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const SplitPtrs& out, const float* sq,
+                             int* niter, const float* guess, const SplitArgs& a, hipStream_t st);
+
+}  // namespace pond

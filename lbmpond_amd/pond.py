@@ -84,6 +84,7 @@ _lib = None
 _D = ctypes.POINTER(ctypes.c_double)
 _I32 = ctypes.POINTER(ctypes.c_int32)
 _I64 = ctypes.POINTER(ctypes.c_int64)
+_F = ctypes.POINTER(ctypes.c_float)
 _CTX = ctypes.c_void_p
 SIGNATURES = {
     "pond_params_default": (None, [ctypes.POINTER(PondParams), ctypes.c_int]),
@@ -103,6 +104,15 @@ SIGNATURES = {
     "pond_drop": (ctypes.c_int, [_CTX, ctypes.c_char_p]),
     "pond_read_arr": (ctypes.c_int, [ctypes.c_char_p, _I64, _D]),
     "pond_device_bytes": (ctypes.c_int64, [_CTX]),
+    "pond_mlp_num_params": (ctypes.c_int64, []),
+    "pond_mlp_init": (ctypes.c_int, [_CTX, ctypes.c_uint64]),
+    "pond_mlp_collect": (ctypes.c_int, [_CTX, ctypes.c_int64, ctypes.c_uint64]),
+    "pond_mlp_dataset_size": (ctypes.c_int, [_CTX, _I64]),
+    "pond_mlp_train": (ctypes.c_int, [_CTX, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, _F]),
+    "pond_mlp_get_params": (ctypes.c_int, [_CTX, _F, _F]),
+    "pond_mlp_set_params": (ctypes.c_int, [_CTX, _F, _F]),
+    "pond_mlp_forward": (ctypes.c_int, [_CTX, _F, ctypes.c_int64, _F]),
+    "pond_mlp_grad": (ctypes.c_int, [_CTX, _F, _F, ctypes.c_int64, _F, _F]),
 }
 
 
@@ -267,6 +277,55 @@ class Pond:
 
     def device_bytes(self):
         return int(lib().pond_device_bytes(self._h))
+
+    # -- learned gauge initializer
+    def mlp_init(self, seed=0):
+        self._check(lib().pond_mlp_init(self._h, int(seed)), "pond_mlp_init")
+
+    def mlp_collect(self, nsamples, seed=0):
+        self._check(lib().pond_mlp_collect(self._h, int(nsamples), int(seed)), "pond_mlp_collect")
+
+    def mlp_dataset_size(self):
+        n = ctypes.c_int64(0)
+        self._check(lib().pond_mlp_dataset_size(self._h, ctypes.byref(n)), "pond_mlp_dataset_size")
+        return int(n.value)
+
+    def mlp_train(self, epochs=4, batch=4096, lr=2e-3, seed=0):
+        losses = np.zeros(int(epochs), np.float32)
+        self._check(lib().pond_mlp_train(self._h, int(epochs), int(batch), float(lr), int(seed),
+                                         losses.ctypes.data_as(_F)), "pond_mlp_train")
+        return losses
+
+    def mlp_params(self):
+        n = int(lib().pond_mlp_num_params())
+        p = np.zeros(n, np.float32)
+        sc = np.zeros(9, np.float32)
+        self._check(lib().pond_mlp_get_params(self._h, p.ctypes.data_as(_F), sc.ctypes.data_as(_F)),
+                    "pond_mlp_get_params")
+        return p, sc
+
+    def mlp_set_params(self, params, scales=None):
+        p = np.ascontiguousarray(params, np.float32)
+        sc = None if scales is None else np.ascontiguousarray(scales, np.float32)
+        self._check(lib().pond_mlp_set_params(self._h, p.ctypes.data_as(_F),
+                                              None if sc is None else sc.ctypes.data_as(_F)), "pond_mlp_set_params")
+
+    def mlp_forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        B = X.shape[0]
+        Y = np.zeros((B, 4), np.float32)
+        self._check(lib().pond_mlp_forward(self._h, X.ctypes.data_as(_F), B, Y.ctypes.data_as(_F)),
+                    "pond_mlp_forward")
+        return Y
+
+    def mlp_grad(self, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        g = np.zeros(int(lib().pond_mlp_num_params()), np.float32)
+        loss = ctypes.c_float(0)
+        self._check(lib().pond_mlp_grad(self._h, X.ctypes.data_as(_F), T.ctypes.data_as(_F), X.shape[0],
+                                        g.ctypes.data_as(_F), ctypes.byref(loss)), "pond_mlp_grad")
+        return g, float(loss.value)
 
 
 # ------------------------------------------------------------------ reference-named facade

@@ -262,6 +262,17 @@ static double lagrpol(int N, int ix, int iy, int iz, double sx, double sy, doubl
 
 static long wrap(long v, long n) { return (v % n + n) % n; }
 
+/* floor(v) as a stencil coordinate: the reference converts with make_int3(floor(...)), whose
+   CUDA double->int conversion maps NaN to 0; values beyond +-2^30 cells (reachable only from
+   non-finite or absurd velocities) are clamped, as the HIP kernels do. */
+static int floor_int(double v) {
+  const double f = floor(v);
+  if (!(f == f)) return 0;
+  if (f > 1073741824.0) return 1073741824;
+  if (f < -1073741824.0) return -1073741824;
+  return (int)f;
+}
+
 static const double* node(const orc_state* S, long x, long y, long z) {
   x = wrap(x, S->p.nx); y = wrap(y, S->p.ny); z = wrap(z, S->p.nz);
   return S->ld + (size_t)(x + S->p.nx * (y + S->p.ny * z)) * S->rec;
@@ -295,8 +306,8 @@ static double lagreval3(const orc_state* S, const double* crd, int lx, int ly, i
    nodes floor(y+0.5)-1..+1 and smoothness samples on the departure point's own row
    (floor(y+0.5), floor(z+0.5)).  Everything else, including the operation order, is shared. */
 static double interp_weno5_impl(const orc_state* S, const double* crd, int i, const double* base, int centred) {
-  const int r = (int)floor(crd[0] + 0.5);
-  const int ry = (int)floor(crd[1] + 0.5), rz = (int)floor(crd[2] + 0.5);
+  const int r = floor_int(crd[0] + 0.5);
+  const int ry = floor_int(crd[1] + 0.5), rz = floor_int(crd[2] + 0.5);
   const int ly = centred ? ry - 1 : ry, lz = centred ? rz - 1 : rz;
   const double S0 = lagreval3(S, crd, r - 2, ly, lz, i, base);
   const double S1 = lagreval3(S, crd, r - 1, ly, lz, i, base);
@@ -327,9 +338,9 @@ static double interp_weno5_impl(const orc_state* S, const double* crd, int i, co
 
 /* propagate<4> (steps.cu:986-1036): tricubic Lagrange, nodes floor(x)-1 .. floor(x)+2 */
 static double interp_lagr4(const orc_state* S, const double* crd, int i, const double* base) {
-  const int lx = (int)(floor(crd[0]) - 4 / 2 + 1);
-  const int ly = (int)(floor(crd[1]) - 4 / 2 + 1);
-  const int lz = (int)(floor(crd[2]) - 4 / 2 + 1);
+  const int lx = floor_int(crd[0]) - 4 / 2 + 1;
+  const int ly = floor_int(crd[1]) - 4 / 2 + 1;
+  const int lz = floor_int(crd[2]) - 4 / 2 + 1;
   const double sx = crd[0] - lx, sy = crd[1] - ly, sz = crd[2] - lz;
   double fpr = 0;
   for (int ix = 0; ix < 4; ix++)

@@ -73,6 +73,13 @@ def lib():
         L.orc_totals.argtypes = [ctypes.c_void_p, _D]
         L.orc_threads.restype = ctypes.c_int
         L.orc_threads.argtypes = []
+        L.orc_mlp_nparam.restype = ctypes.c_int
+        L.orc_mlp_nparam.argtypes = []
+        _F = ctypes.POINTER(ctypes.c_float)
+        L.orc_mlp_forward.restype = None
+        L.orc_mlp_forward.argtypes = [_F, ctypes.c_int64, _F, _F, ctypes.c_int]
+        L.orc_mlp_grad.restype = None
+        L.orc_mlp_grad.argtypes = [_F, ctypes.c_int64, _F, _F, _D, _D, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -112,6 +119,33 @@ def moments(name, f, gauge):
 
 def threads():
     return int(lib().orc_threads())
+
+
+def _f32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+def mlp_nparam():
+    return int(lib().orc_mlp_nparam())
+
+
+def mlp_forward(params, X, bf16=True):
+    """network output for normalized features X[B][135] (oracle/mlp_oracle.c)"""
+    P, X = _f32(params), _f32(X)
+    Y = np.zeros((X.shape[0], 4), np.float32)
+    F = ctypes.POINTER(ctypes.c_float)
+    lib().orc_mlp_forward(P.ctypes.data_as(F), X.shape[0], X.ctypes.data_as(F), Y.ctypes.data_as(F), int(bf16))
+    return Y
+
+
+def mlp_grad(params, X, T, bf16=True):
+    P, X, T = _f32(params), _f32(X), _f32(T)
+    g = np.zeros(P.size)
+    loss = np.zeros(1)
+    F = ctypes.POINTER(ctypes.c_float)
+    lib().orc_mlp_grad(P.ctypes.data_as(F), X.shape[0], X.ctypes.data_as(F), T.ctypes.data_as(F), _p(g), _p(loss),
+                       int(bf16))
+    return g, float(loss[0])
 
 
 class Oracle:

@@ -1,0 +1,75 @@
+"""BF16 MFMA network kernels against the network oracle (same BF16 rounding points, FP32/FP64
+accumulation), training on solver-generated data, and what the learned starting gauge gains."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from helpers import FIELDS, rel_linf
+from lbmpond_amd import Pond
+from lbmpond_amd import ic
+from test_mlp_oracle import make
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mfma_forward_matches_oracle(gpu):
+    p, X, _ = make(11, B=200)  # 200: not a multiple of the 16-cell tile
+    with Pond("D3Q27", (4, 4, 4), "weno5c", "split", use_mlp=True) as pond:
+        pond.mlp_set_params(p, np.array([1, 1, 1, 1, 1, 1, 1, 1, 1], np.float32))
+        y = pond.mlp_forward(X)
+    ref = po.mlp_forward(p, X, bf16=True)
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    print("forward rel Linf vs oracle:", err)
+    assert err < 2e-3
+
+
+def test_mfma_gradients_match_oracle(gpu):
+    p, X, T = make(12, B=256)
+    with Pond("D3Q27", (4, 4, 4), "weno5c", "split", use_mlp=True) as pond:
+        pond.mlp_set_params(p, np.ones(9, np.float32))
+        g, loss = pond.mlp_grad(X, T)
+    gr, lr = po.mlp_grad(p, X, T, bf16=True)
+    print("loss gpu", loss, "oracle", lr)
+    assert abs(loss - lr) <= 1e-4 * lr
+    blocks = {"W1": (0, 8640), "b1": (8640, 8704), "W2": (8704, 12800), "b2": (12800, 12864), "W3": (12864, 13120),
+              "b3": (13120, 13124)}
+    for name, (a, b) in blocks.items():
+        e = rel_linf(g[a:b], gr[a:b])
+        print(f"  grad {name}: rel Linf {e:.2e}")
+        assert e < 1e-2, name
+
+
+def test_learned_start_reduces_iterations(gpu):
+    """Train on the solver's own steps of a small shock-bubble case, then continue the same
+    state with and without the learned starting gauge: fewer gauge iterations, and both stay
+    within the split path's error of each other."""
+    case = ic.shock_bubble(n=32)
+    with Pond("D3Q27", case.shape, "weno5c", "split", use_mlp=True, dtau=case.dtau) as p:
+        p.set_state(*case.fields)
+        for s in range(12):
+            p.step(1)
+            p.mlp_collect(4096, seed=100 + s)
+        losses = p.mlp_train(epochs=6, batch=1024, lr=2e-3, seed=7)
+        print("training losses per epoch:", losses)
+        assert losses[-1] < losses[0]
+        f = p.populations()
+        st = p.fields()
+        params, scales = p.mlp_params()
+        iters_mlp = []
+        for _ in range(6):
+            p.step(1)
+            iters_mlp.append(p.diagnostics()["niter_mean"])
+        out_mlp = p.fields()
+    with Pond("D3Q27", case.shape, "weno5c", "split", dtau=case.dtau) as q:
+        q.set_populations(f.reshape(-1), st["rho"], st["ux"], st["uy"], st["uz"], st["T"])
+        iters_plain = []
+        for _ in range(6):
+            q.step(1)
+            iters_plain.append(q.diagnostics()["niter_mean"])
+        out_plain = q.fields()
+    print("mean gauge iterations: learned", np.mean(iters_mlp), "plain", np.mean(iters_plain))
+    assert np.mean(iters_mlp) < np.mean(iters_plain)
+    for k in FIELDS:
+        e = rel_linf(out_mlp[k], out_plain[k])
+        print(f"  learned vs plain start {k}: {e:.2e}")
+        assert e < 1e-4

@@ -1,0 +1,78 @@
+"""FP32/FP64-split step against the plain FP64 step (the FP64 oracle run: the plain path is
+bit-identical to the CPU oracle, see test_parity_gpu.py) on the same seeded inputs.
+
+The bar of the north star is <= 1e-5 relative L-inf on rho, u, T; every test prints the
+measured error and the mean gauge iterations of both paths.
+"""
+import numpy as np
+import pytest
+
+from helpers import FIELDS, rel_linf, smooth_random_state
+from lbmpond_amd import Pond
+from lbmpond_amd import ic
+
+pytestmark = pytest.mark.gpu
+
+
+def run(lattice, shape, interp, precision, fields, steps, **kw):
+    with Pond(lattice, shape, interp, precision, **kw) as p:
+        p.set_state(*fields)
+        iters = []
+        for _ in range(steps):
+            p.step(1)
+            iters.append(p.diagnostics()["niter_mean"])
+        out = p.fields()
+        out["niter_hist"] = np.array(iters)
+        return out
+
+
+def compare(shape, interp, fields, steps, tol, dtau=1.0 / 0.9, lattice="D3Q27"):
+    print(f"[split] {lattice} {interp} {shape} steps={steps}", flush=True)
+    ref = run(lattice, shape, interp, "fp64", fields, steps, dtau=dtau)
+    got = run(lattice, shape, interp, "split", fields, steps, dtau=dtau)
+    errs = {k: rel_linf(got[k], ref[k]) for k in FIELDS}
+    print("  split vs fp64 rel Linf:", {k: f"{v:.2e}" for k, v in errs.items()},
+          f" mean iters fp64 {ref['niter_hist'].mean():.2f} split {got['niter_hist'].mean():.2f}", flush=True)
+    for k in FIELDS:
+        assert errs[k] <= tol, (k, errs[k])
+    return errs, ref, got
+
+
+def test_split_weno5c_random(gpu):
+    shape = (16, 12, 10)
+    compare(shape, "weno5c", smooth_random_state(shape, seed=61, T0=0.01), 20, 1e-5)
+
+
+def test_split_lagr4_random(gpu):
+    shape = (16, 12, 10)
+    compare(shape, "lagr4", smooth_random_state(shape, seed=62, T0=0.02), 20, 1e-5)
+
+
+def test_split_taylor_green_small(gpu):
+    case = ic.taylor_green(n=32, steps=50)
+    compare(case.shape, "weno5c", case.fields, 50, 1e-5, dtau=case.dtau)
+
+
+def test_split_vortex_small(gpu):
+    case = ic.vortex_3d(n=32)
+    compare(case.shape, "weno5c", case.fields, 30, 1e-5)
+
+
+def test_split_fewer_or_equal_iterations(gpu):
+    shape = (16, 16, 16)
+    f = smooth_random_state(shape, seed=63, T0=0.01)
+    _, ref, got = compare(shape, "weno5c", f, 10, 1e-5)
+    assert got["niter_hist"].mean() <= ref["niter_hist"].mean()
+
+
+def test_split_state_roundtrip(gpu):
+    shape = (8, 4, 4)
+    rho, ux, uy, uz, T = smooth_random_state(shape, seed=64, T0=0.02)
+    with Pond("D3Q27", shape, "weno5c", "split") as p:
+        p.set_state(rho, ux, uy, uz, T)
+        out = p.fields()
+        f = p.populations()
+    assert rel_linf(out["rho"], rho) < 1e-7 and rel_linf(out["ux"], ux) < 1e-7
+    assert np.array_equal(out["T"], T)  # T is kept in FP64
+    w = np.array([8 / 27] + [0] * 26)
+    assert abs(f[0] - (w[0] * rho).astype(np.float32)).max() < 1e-6
