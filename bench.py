@@ -1,18 +1,24 @@
 #!/usr/bin/env python3
 """PonD step benchmark: prints ONE JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c4|c5] [--n N]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c4] [--n N] [--no-cpu]
 
-A "step" is one PonD step (gauge fixed-point iteration + pull streaming + BGK collide) over
-the whole grid.  Inputs are resident in HBM before the timed region; the timed region is
-K steps bracketed by device synchronisation.  MLUPS = cells * K / wall time of the timed
-region.  The step kernel's own launch time is measured with HIP events recorded on the
-library's stream around every launch (pond_timing_info), and the roofline uses it.
+Headline (BASELINE metric): MLUPS and mean gauge iterations per step of the learned-initializer
++ FP32/FP64-split PonD step against the plain FP64 step on D3Q27 384^3, with the relative
+L-inf error of the split paths against the FP64 run on the same inputs after the same steps.
 
-The CPU baseline is the oracle (the restatement of the reference's step, no memoisation)
-timed on the host cores on a bounded sample of the same initial-condition family.
+Each path starts from the same seeded initial condition and takes C + W + K steps: C
+"collect" steps (the learned path samples its training data from them and trains its
+network after them; the other paths just step), W untimed warmup steps, then K timed steps.
+A step is one PonD step over the whole grid (gauge fixed-point iteration, pull streaming,
+BGK collide; for the learned path also the network inference that predicts the starting
+gauge).  Inputs are resident in HBM before the timed region, which is bracketed by device
+synchronisation; MLUPS = cells * K / wall time.  The step kernels' own times come from HIP
+events recorded on the library's stream around every launch (pond_timing_info).
 
-Only one GPU is used; --gpus above 1 exits with a message.
+The CPU baseline is the oracle (the restatement of the reference's step, as written, no
+memoisation) timed on the host threads on a bounded sample of the same initial-condition
+family.  Only one GPU is used; --gpus above 1 exits with a message.
 """
 import argparse
 import json
@@ -28,12 +34,13 @@ import numpy as np  # noqa: E402
 METRIC = "MLUPS + mean gauge iters/step, learned+FP32 vs plain FP64, at stated L∞ err"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FIELDS = ("rho", "ux", "uy", "uz", "T")
 
-# algorithmic bytes per lattice update (compulsory HBM traffic): read one state, write one
-# state and the int32 iteration count: 2*(Qn*8 + 5*8) + 4
-BYTES_PER_LUP_F64 = {"D1Q5": 2 * (5 * 8 + 40) + 4, "D3Q27": 2 * (27 * 8 + 40) + 4, "D3Q125": 2 * (125 * 8 + 40) + 4}
+# algorithmic bytes per lattice update: read one state, write one state, int32 iterations
+BYTES_PER_LUP = {("D3Q27", "fp64"): 2 * (27 * 8 + 5 * 8) + 4, ("D3Q27", "split"): 2 * (27 * 4 + 4 * 4 + 8) + 4,
+                 ("D3Q125", "fp64"): 2 * (125 * 8 + 5 * 8) + 4, ("D1Q5", "fp64"): 2 * (5 * 8 + 5 * 8) + 4}
 # algorithmic FP64 flops per cell-iteration of the reference's algorithm with de-duplicated
-# transfers (19 flops per gauge-transfer term, D3Q27 WENO5: 27 pops x 50 transfers x 27 terms)
+# transfers: 19 flops per D3Q27 transfer term, 27 targets x 50 transfers x 27 terms (WENO5)
 FLOP_PER_CELL_ITER = {"D3Q27": 27 * 50 * 27 * 19, "D3Q125": 125 * 27 * 125 * 29, "D1Q5": 5 * 5 * 5 * 30}
 
 
@@ -59,50 +66,72 @@ def interp_for(case):
     return "weno5c" if case.lattice == "D3Q27" else case.interp
 
 
-def time_gpu(case, precision, steps, warmup, **kw):
+def rel_linf(a, b):
+    den = np.abs(b).max()
+    return float(np.abs(a - b).max() / den) if den > 0 else float(np.abs(a - b).max())
+
+
+def run_path(case, precision, learned, collect, warmup, steps, train_cfg):
     from lbmpond_amd import Pond
     interp = interp_for(case)
-    p = Pond(case.lattice, case.shape, interp, precision, dtau=case.dtau, **kw)
+    tag = f"{precision}{'+learned' if learned else ''}"
+    p = Pond(case.lattice, case.shape, interp, precision, dtau=case.dtau, use_mlp=learned)
     p.set_state(*case.fields)
-    log(f"[bench] {case.name} {case.lattice} {case.shape} {interp} {precision}: warmup {warmup}")
-    for _ in range(warmup):
+    res = {"path": tag}
+    for c in range(collect):
+        p.step(1)
+        if learned:
+            p.mlp_collect(train_cfg["samples_per_step"], seed=1000 + c)
+        p.sync()
+        log(f"[bench] {tag}: collect step {c + 1}/{collect}")
+    if learned:
+        t = time.perf_counter()
+        losses = p.mlp_train(epochs=train_cfg["epochs"], batch=train_cfg["batch"], lr=train_cfg["lr"], seed=11)
+        res["train_s"] = round(time.perf_counter() - t, 3)
+        res["train_samples"] = p.mlp_dataset_size()
+        res["train_loss"] = [round(float(x), 5) for x in losses]
+        log(f"[bench] {tag}: trained on {res['train_samples']} samples in {res['train_s']} s, losses {res['train_loss']}")
+    for w in range(warmup):
         p.step(1)
         p.sync()
+        log(f"[bench] {tag}: warmup step {w + 1}/{warmup}")
     t0 = p.timing()
+    iters = []
     p.sync()
     w0 = time.perf_counter()
     for k in range(steps):
         p.step(1)
         p.sync()
-        log(f"[bench] step {k + 1}/{steps} done")
-    w1 = time.perf_counter()
+        iters.append(p.diagnostics()["niter_mean"])
+        log(f"[bench] {tag}: timed step {k + 1}/{steps}")
+    wall = time.perf_counter() - w0
     t1 = p.timing()
-    d = p.diagnostics()
     cells = int(np.prod(case.shape))
-    wall_s = w1 - w0
-    kern_ms = t1["total_ms"] - t0["total_ms"]
-    res = dict(p=p, cells=cells, wall_s=wall_s, kernel_ms=kern_ms, steps=steps, diag=d,
-               mlups=cells * steps / wall_s / 1e6, kernel_mlups=cells * steps / (kern_ms * 1e-3) / 1e6)
+    res.update(cells=cells, wall_s=wall, kernel_ms=t1["total_ms"] - t0["total_ms"], mlp_ms=t1["last_mlp_ms"],
+               mlups=cells * steps / wall / 1e6, niter_mean=float(np.mean(iters)),
+               niter_max=int(p.diagnostics()["niter_max"]))
+    res["fields"] = p.fields()
+    p.close()
     return res
 
 
-def cpu_baseline(case_name, lattice, interp):
+def cpu_baseline(case_name, interp):
     """oracle on a bounded sample of the same IC family, all host threads"""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     sample_n = {"c2": 24, "c3": 24, "c4": 24, "c5": 8}[case_name]
-    steps = 1
     case = make_case(case_name, sample_n)
     o = pyoracle.Oracle(case.lattice, case.shape, interp, dtau=case.dtau)
     o.set_state(*case.fields)
     t = time.perf_counter()
-    o.step(steps)
+    o.step(1)
     dt = time.perf_counter() - t
     cells = int(np.prod(case.shape))
     niter = float(o.fields()["niter"].mean())
-    return {"value": cells * steps / dt / 1e6, "unit": "MLUPS", "cores": pyoracle.threads(), "kind": "port",
-            "sample": f"{case.lattice} {interp} {sample_n}^3 cells x {steps} step of the {case_name} initial "
-                      f"condition family ({cells * steps} cell-steps, {dt:.1f} s, mean gauge iterations {niter:.2f})"}
+    return {"value": round(cells / dt / 1e6, 6), "unit": "MLUPS", "cores": pyoracle.threads(), "kind": "port",
+            "sample": f"oracle (reference algorithm as written, no memoisation), {case.lattice} {interp} "
+                      f"{sample_n}^3 cells x 1 step of the {case_name} initial-condition family "
+                      f"({cells} cell-steps in {dt:.1f} s, mean gauge iterations {niter:.2f})"}
 
 
 def main():
@@ -110,57 +139,81 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2")
+    ap.add_argument("--collect", type=int, default=4, help="steps whose cells train the network")
+    ap.add_argument("--config", default="c4")
     ap.add_argument("--n", type=int, default=None, help="grid edge override (testing only)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--paths", default="fp64,split,learned")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU sharding is described in DESIGN.md "
-              "and not launched (use --gpus 1)", file=sys.stderr)
+        print("bench.py: this build runs on one GPU only; how the step would shard is described in DESIGN.md; "
+              "use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     case = make_case(args.config, args.n)
-    log(f"[bench] building initial condition done: {case.name} {case.shape}")
-    r64 = time_gpu(case, "fp64", args.steps, args.warmup)
-    diag = r64["diag"]
-    niter = float(diag["niter_mean"])
-    cells = r64["cells"]
-    launch_s = r64["kernel_ms"] * 1e-3 / args.steps
-    bpl = BYTES_PER_LUP_F64[case.lattice]
-    achieved_gbs = bpl * cells / launch_s / 1e9
-    flops = FLOP_PER_CELL_ITER[case.lattice] * niter * cells
+    cells = int(np.prod(case.shape))
+    log(f"[bench] initial condition ready: {case.name} {case.lattice} {case.shape}")
+    train_cfg = {"samples_per_step": max(2048, min(500_000, cells // 8)), "epochs": 3, "batch": 4096, "lr": 2e-3}
+    paths = args.paths.split(",")
+    results = {}
+    for name in paths:
+        prec = "fp64" if name == "fp64" else "split"
+        results[name] = run_path(case, prec, name == "learned", args.collect, args.warmup, args.steps, train_cfg)
+
+    ref = results.get("fp64")
+    head = results.get("learned") or results.get("split") or ref
+    summary = {}
+    for name, r in results.items():
+        s = {"mlups": round(r["mlups"], 4), "niter_mean": round(r["niter_mean"], 4), "niter_max": r["niter_max"],
+             "ms_per_step": round(r["wall_s"] * 1e3 / args.steps, 3),
+             "kernel_ms_per_step": round(r["kernel_ms"] / args.steps, 3)}
+        if ref is not None and name != "fp64":
+            s["rel_linf_vs_fp64"] = {k: float(f"{rel_linf(r['fields'][k], ref['fields'][k]):.3e}") for k in FIELDS}
+        for k in ("train_s", "train_samples", "train_loss"):
+            if k in r:
+                s[k] = r[k]
+        if r.get("mlp_ms"):
+            s["mlp_infer_ms"] = round(r["mlp_ms"], 3)
+        summary[name] = s
+    head_name = [k for k, v in results.items() if v is head][0]
+    prec = "fp64" if head_name == "fp64" else "split"
+    launch_s = head["kernel_ms"] * 1e-3 / args.steps
+    bpl = BYTES_PER_LUP[(case.lattice, prec)]
+    achieved = bpl * cells / launch_s / 1e9
+    flops = FLOP_PER_CELL_ITER[case.lattice] * head["niter_mean"] * cells
+    err = summary[head_name].get("rel_linf_vs_fp64")
     out = {
         "metric": METRIC,
-        "value": round(r64["mlups"], 4),
+        "value": round(head["mlups"], 4),
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(r64["wall_s"] * 1e3 / args.steps, 3),
+        "ms_per_step": round(head["wall_s"] * 1e3 / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if prec == "split" else "f64",
         "data": "synthetic",
-        "config": {"workload": f"{case.name}: {case.lattice} {'x'.join(map(str, case.shape))} "
-                               f"{interp_for(case)} plain FP64 PonD step", "lattice": case.lattice,
-                   "grid": list(case.shape), "interp": interp_for(case), "precision": "fp64",
-                   "tol_rel": 1e-10, "tol_abs": 1e-12, "dtau": case.dtau},
-        "niter_mean": round(niter, 4),
-        "niter_max": int(diag["niter_max"]),
-        "kernel_ms_per_step": round(r64["kernel_ms"] / args.steps, 3),
-        "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 6), "traffic": None,
-                     "bytes_per_lup": bpl},
-        "valu_roofline": {"bound": "fp64-valu", "achieved": round(flops / launch_s / 1e12, 4),
-                          "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                          "frac": round(flops / launch_s / 1e12 / FP64_PEAK_TFLOPS, 5),
-                          "flop_per_cell_iter": FLOP_PER_CELL_ITER[case.lattice]},
+        "config": {"workload": f"{case.name}: {case.lattice} {'x'.join(map(str, case.shape))} {interp_for(case)} "
+                               f"PonD step, path {head_name}", "lattice": case.lattice, "grid": list(case.shape),
+                   "interp": interp_for(case), "dtau": case.dtau, "collect_steps": args.collect,
+                   "paths": paths},
+        "niter_mean": round(head["niter_mean"], 4),
+        "rel_linf_vs_fp64": err,
+        "speedup_vs_fp64": round(head["mlups"] / ref["mlups"], 3) if ref is not None and head is not ref else None,
+        "paths": summary,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "bytes_per_lup": bpl,
+                     "kernel": "k_step_split27<2>" if prec == "split" else "k_step_f64<1,2>"},
+        "valu_roofline": {"bound": "fp64-valu", "achieved": round(flops / launch_s / 1e12, 4), "peak": FP64_PEAK_TFLOPS,
+                          "unit": "TFLOP/s", "frac": round(flops / launch_s / 1e12 / FP64_PEAK_TFLOPS, 5),
+                          "flop_per_cell_iter": FLOP_PER_CELL_ITER[case.lattice],
+                          "note": "reference-algorithm FP64 flops at the path's mean iterations"},
     }
-    r64["p"].close()
     if not args.no_cpu:
         log("[bench] cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(case.name, case.lattice, interp_for(case))
+        out["cpu_baseline"] = cpu_baseline(case.name, interp_for(case))
     print(json.dumps(out), flush=True)
 
 

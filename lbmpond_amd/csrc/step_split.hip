@@ -14,7 +14,8 @@
 //
 // Convergence (FP32-appropriate; the reference's 1e-10 relative test cannot be met by FP32
 // populations): the FP64 gauge residual |du_d| <= tol_rel*sqrt(T) + tol_abs and
-// |dT| <= tol_rel*T + tol_abs.  Populations are a deterministic function of the gauge, so they
+// |dT| <= tol_rel*T + tol_abs (defaults 1e-7, 1e-12: FP32 populations leave a noise floor near
+// 3e-8 on this residual).  Populations are a deterministic function of the gauge, so they
 // converge with it.
 //
 // Starting gauge: the cell's previous gauge, or (learned initializer) a predicted gauge.  The
