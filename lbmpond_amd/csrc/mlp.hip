@@ -59,7 +59,7 @@ __device__ __forceinline__ float raw_feature(const MlpState& s, int x, int y, in
 // One wave per 16-cell tile loop.  Features are taken from the state (mode 0) or from a
 // normalized feature matrix (mode 1, for checking the network arithmetic).
 template <int MODE>
-__global__ __launch_bounds__(64) void k_mlp_infer(const MlpWeightsBF w, MlpState s, const float* __restrict__ Xin,
+__global__ __launch_bounds__(256) void k_mlp_infer(const MlpWeightsBF w, MlpState s, const float* __restrict__ Xin,
                                                   int64_t ncell, float* __restrict__ out, int64_t out_plane) {
   const int l = threadIdx.x & 63;
   const int q = l >> 4, col = l & 15;

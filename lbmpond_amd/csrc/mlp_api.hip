@@ -348,16 +348,17 @@ pond_status pond_mlp_forward(pond_ctx* ctx, const float* X, int64_t B, float* Y)
   float *dX = nullptr, *dY = nullptr;
   MLP_HIP(ctx, hipMalloc(&dX, (size_t)B * kMlpIn * 4));
   MLP_HIP(ctx, hipMalloc(&dY, (size_t)B * kMlpOut * 4));
-  pond_status st = POND_OK;
-  if (hipMemcpy(dX, X, (size_t)B * kMlpIn * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      mlp_launch_infer_matrix(pond_mlp_weights(ctx), dX, B, dY, ctx->stream) != hipSuccess ||
-      hipStreamSynchronize(ctx->stream) != hipSuccess)
-    st = POND_ERR_HIP;
   std::vector<float> tmp((size_t)B * kMlpOut);
-  if (st == POND_OK && hipMemcpy(tmp.data(), dY, tmp.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) st = POND_ERR_HIP;
+  hipError_t e = hipMemcpy(dX, X, (size_t)B * kMlpIn * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = mlp_launch_infer_matrix(pond_mlp_weights(ctx), dX, B, dY, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = hipMemcpy(tmp.data(), dY, tmp.size() * 4, hipMemcpyDeviceToHost);
   (void)hipFree(dX);
   (void)hipFree(dY);
-  if (st != POND_OK) { ctx->err = "pond_mlp_forward failed"; return st; }
+  if (e != hipSuccess) {
+    ctx->err = std::string("pond_mlp_forward: ") + hipGetErrorString(e);
+    return POND_ERR_HIP;
+  }
   // output planes [o][b] -> rows [b][o]
   for (int64_t b = 0; b < B; b++)
     for (int o = 0; o < kMlpOut; o++) Y[b * kMlpOut + o] = tmp[(size_t)o * B + b];

@@ -233,6 +233,9 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
   if (!make_lattice(p->lattice, &ctx->L)) return bail(POND_ERR_INVALID, "pond_create: unknown lattice");
   if (p->nx < 1 || p->ny < 1 || p->nz < 1 || p->nx > (1 << 20) || p->ny > (1 << 20) || p->nz > (1 << 20))
     return bail(POND_ERR_INVALID, "pond_create: grid sizes must be in [1, 2^20]");
+  // node data are addressed with 32-bit byte offsets into 8-byte planes
+  if (p->nx * p->ny * p->nz > ((int64_t)1 << 29) - 64)
+    return bail(POND_ERR_INVALID, "pond_create: at most 2^29 - 64 cells per context");
   if (p->interp != POND_WENO5 && p->interp != POND_LAGR4 && p->interp != POND_WENO5C) return bail(POND_ERR_INVALID, "pond_create: unknown interp");
   if (p->precision != POND_FP64 && p->precision != POND_SPLIT)
     return bail(POND_ERR_INVALID, "pond_create: unknown precision");
@@ -270,6 +273,10 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
         return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of the state buffers failed");
       ctx->bytes += (int64_t)(fbytes + mbytes);
     }
+    if (hipMalloc(&ctx->nodeP, (size_t)g.plane * sizeof(double)) != hipSuccess ||
+        hipMalloc(&ctx->nodeok, (size_t)g.plane) != hipSuccess)
+      return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of the node planes failed");
+    ctx->bytes += (int64_t)g.plane * 9;
   } else {
     const size_t fbytes = (size_t)ctx->Qn * g.plane * sizeof(float);
     const size_t mbytes = (size_t)4 * g.plane * sizeof(float);
@@ -329,6 +336,8 @@ void pond_destroy(pond_ctx* ctx) {
     if (e) (void)hipEventDestroy(e);
   if (ctx->niter) (void)hipFree(ctx->niter);
   if (ctx->sq) (void)hipFree(ctx->sq);
+  if (ctx->nodeP) (void)hipFree(ctx->nodeP);
+  if (ctx->nodeok) (void)hipFree(ctx->nodeok);
   if (ctx->guess) (void)hipFree(ctx->guess);
   if (ctx->red) (void)hipFree(ctx->red);
   for (auto& e : ctx->ev_pool) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
@@ -468,8 +477,11 @@ pond_status pond_step(pond_ctx* ctx, int nsteps) {
       he = launch_step_split(ctx->p.lattice, ctx->p.interp, pin, pout, ctx->sq, ctx->niter,
                              learned ? ctx->guess : nullptr, ctx->sargs, ctx->stream);
     } else {
-      he = launch_step_f64(ctx->p.lattice, ctx->p.interp, ctx->f64[in].f, ctx->f64[in].m, ctx->f64[outb].f,
-                           ctx->f64[outb].m, ctx->niter, ctx->args, ctx->stream);
+      he = launch_node_prep(ctx->p.lattice, ctx->f64[in].f, ctx->f64[in].m, ctx->nodeP, ctx->nodeok, ctx->g.n,
+                            ctx->g.plane, ctx->stream);
+      if (he == hipSuccess)
+        he = launch_step_f64(ctx->p.lattice, ctx->p.interp, ctx->f64[in].f, ctx->f64[in].m, ctx->nodeP,
+                             ctx->nodeok, ctx->f64[outb].f, ctx->f64[outb].m, ctx->niter, ctx->args, ctx->stream);
     }
     if (he != hipSuccess) return fail(ctx, POND_ERR_HIP, std::string("step kernel launch: ") + hipGetErrorString(he));
     POND_HIP(ctx, hipEventRecord(e.b, ctx->stream));

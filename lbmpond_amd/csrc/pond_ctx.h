@@ -56,8 +56,11 @@ struct MlpCtx {
   int* idx = nullptr;
 };
 
-hipError_t launch_step_f64(int lattice, int interp, const double* fin, const double* min, double* fout,
-                           double* mout, int* niter, const StepArgs& a, hipStream_t st);
+hipError_t launch_node_prep(int lattice, const double* f, const double* m, double* P, unsigned char* ok, int64_t n,
+                            int64_t plane, hipStream_t st);
+hipError_t launch_step_f64(int lattice, int interp, const double* fin, const double* min, const double* Pin,
+                           const unsigned char* okin, double* fout, double* mout, int* niter, const StepArgs& a,
+                           hipStream_t st);
 
 }  // namespace pond
 
@@ -74,6 +77,8 @@ struct pond_ctx {
   pond::F64Buf f64[2];
   pond::SplitBuf sp[2];
   pond::SplitArgs sargs{};
+  double* nodeP = nullptr;  // plain path: sqrt(|T|)*sign(T) per cell of the load buffer
+  unsigned char* nodeok = nullptr;  // plain path: populations finite and below 2^200
   float* sq = nullptr;      // split path: sqrt(|T|)*sign(T) per cell of the load buffer
   float* guess = nullptr;   // split path: learned starting gauge increments (4 planes)
   int* niter = nullptr;     // per-cell gauge iterations of the last step

@@ -29,7 +29,11 @@ struct Tgt27 {
   double W;           // (wk*wl)*wm
   double Cx, Cy, Cz;  // (e_inew*sqrt(|T1|))*sign(T1)
   bool kx, ky, kz;    // e_inew component != 0
+  bool wok;           // |W| < 2^200
 };
+
+// magnitude bound of the fast path: five factors below 2^200 multiply to below 2^1000
+constexpr double kSafe = 1.6069380442589903e60;  // 2^200
 
 __device__ __forceinline__ Tgt27 make_tgt27(int inew, double u1x, double u1y, double u1z, double T1) {
   Tgt27 t;
@@ -46,41 +50,85 @@ __device__ __forceinline__ Tgt27 make_tgt27(int inew, double u1x, double u1y, do
   t.Cy = (double)ey * sq1 * sg1;
   t.Cz = (double)ez * sq1 * sg1;
   t.kx = ex != 0; t.ky = ey != 0; t.kz = ez != 0;
+  t.wok = fabs(t.W) < kSafe;
   return t;
 }
 
-// f: population planes at the node's cell index (stride `plane`); m: moment planes likewise
-__device__ __forceinline__ double transfer27(const double* __restrict__ f, const double* __restrict__ m,
-                                             int64_t plane, const Tgt27& t, double sqTb) {
-  const double u0x = m[1 * plane], u0y = m[2 * plane], u0z = m[3 * plane], T0 = m[4 * plane];
+// Node data of the load state: plane bases plus a per-node 32-bit byte offset.
+struct NodeSrc {
+  const double* __restrict__ f;          // population planes, stride g.plane
+  const double* __restrict__ m;          // rho, ux, uy, uz, T planes
+  const double* __restrict__ P;          // sqrt(|T|)*sign(T) per cell (params.h:110-112)
+  const unsigned char* __restrict__ ok;  // populations finite and below 2^200
+  Grid g;
+};
+
+__device__ __forceinline__ double ldp(const double* base, int64_t k, int64_t plane, uint32_t boff) {
+  return *(const double*)((const char*)(base + k * plane) + boff);
+}
+
+__device__ __forceinline__ uint32_t node_off(const Grid& g, int x, int y, int z) {
+  return (uint32_t)cell_index(g, x, y, z) * 8u;
+}
+
+// basis values of the node at byte offset boff for target t (structs.cuh:530-538).
+// e_iold*sqrt(|T0|)*sign(T0) for e_iold = -1, 0, +1 is exactly {-P, 0*P, P}.
+__device__ __forceinline__ bool basis27(const NodeSrc& s, uint32_t boff, const Tgt27& t, double sqTb, double gx[3],
+                                        double gy[3], double gz[3]) {
+  const int64_t plane = s.g.plane;
+  const double u0x = ldp(s.m, 1, plane, boff), u0y = ldp(s.m, 2, plane, boff), u0z = ldp(s.m, 3, plane, boff);
+  const double Pn = ldp(s.P, 0, plane, boff);
+  const bool nok = s.ok[boff >> 3] != 0;
   const double Dx = (t.u1x - u0x) * sqTb;
   const double Dy = (t.u1y - u0y) * sqTb;
   const double Dz = (t.u1z - u0z) * sqTb;
-  const double sq0 = sqrt(fabs(T0));
-  const double sg0 = (double)isign_d(T0);
-  // e_iold*sqrt(|T0|)*sign(T0) for e_iold = -1, 0, +1
-  const double E[3] = {-1.0 * sq0 * sg0, 0.0 * sq0 * sg0, 1.0 * sq0 * sg0};
-  double gx[3], gy[3], gz[3];
+  const double E[3] = {-Pn, 0.0 * Pn, Pn};
+  bool safe = nok && t.wok;
 #pragma unroll
   for (int j = 0; j < 3; j++) {
     const double ax = Dx - E[j], ay = Dy - E[j], az = Dz - E[j];
     gx[j] = ax * ax - (t.kx ? t.Cx * ax : t.T1);
     gy[j] = ay * ay - (t.ky ? t.Cy * ay : t.T1);
     gz[j] = az * az - (t.kz ? t.Cz * az : t.T1);
+    safe = safe && fabs(gx[j]) < kSafe && fabs(gy[j]) < kSafe && fabs(gz[j]) < kSafe;
   }
-  double p1[3], p2[3][3];
+  return safe;
+}
+
+// the reference's term loop with NaN/Inf terms dropped (structs.cuh:539-542); kept out of line,
+// it runs only for nodes that fail the magnitude test
+__device__ __noinline__ double transfer27_checked(const double* __restrict__ f, int64_t plane, uint32_t boff,
+                                                  double W, const double* gx, const double* gy, const double* gz) {
+  double p2[3][3];
 #pragma unroll
-  for (int a = 0; a < 3; a++) {
-    p1[a] = t.W * gx[a];
+  for (int a = 0; a < 3; a++)
 #pragma unroll
-    for (int b = 0; b < 3; b++) p2[a][b] = p1[a] * gy[b];
-  }
+    for (int b = 0; b < 3; b++) p2[a][b] = W * gx[a] * gy[b];
   double fnew = 0;
 #pragma unroll
   for (int iold = 0; iold < 27; iold++) {
     const int jx = kE27[iold][0] + 1, jy = kE27[iold][1] + 1, jz = kE27[iold][2] + 1;
-    const double fadd = p2[jx][jy] * gz[jz] * f[(int64_t)iold * plane];
+    const double fadd = p2[jx][jy] * gz[jz] * ldp(f, iold, plane, boff);
     if (finite_d(fadd)) fnew += fadd;
+  }
+  return fnew;
+}
+
+// gauge transfer of the node at byte offset boff to target t (structs.cuh:525-548)
+__device__ __forceinline__ double transfer27(const NodeSrc& s, uint32_t boff, const Tgt27& t, double sqTb) {
+  double gx[3], gy[3], gz[3];
+  const bool safe = basis27(s, boff, t, sqTb, gx, gy, gz);
+  if (!safe) return transfer27_checked(s.f, s.g.plane, boff, t.W, gx, gy, gz);
+  double p2[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) p2[a][b] = t.W * gx[a] * gy[b];
+  double fnew = 0;
+#pragma unroll
+  for (int iold = 0; iold < 27; iold++) {
+    const int jx = kE27[iold][0] + 1, jy = kE27[iold][1] + 1, jz = kE27[iold][2] + 1;
+    fnew += p2[jx][jy] * gz[jz] * ldp(s.f, iold, s.g.plane, boff);
   }
   return fnew;
 }
@@ -115,10 +163,10 @@ __device__ __forceinline__ Tgt5 make_tgt5(int inew, double u1x, double T1) {
   return t;
 }
 
-__device__ __forceinline__ double transfer5(const double* __restrict__ f, const double* __restrict__ m,
-                                            int64_t plane, const Tgt5& t) {
-  const double u0x = m[1 * plane], T0 = m[4 * plane];
-  const double s0 = sqrts_d(T0 / 1.0);
+__device__ __forceinline__ double transfer5(const NodeSrc& s, uint32_t boff, const Tgt5& t) {
+  const int64_t plane = s.g.plane;
+  const double u0x = ldp(s.m, 1, plane, boff);
+  const double s0 = ldp(s.P, 0, plane, boff);  // sqrts(T0/1.0) = sqrt(|T0|)*sign(T0)
   const double et = t.et;
   const double difu = t.u1x - u0x;
   double fnew = t.fnew0;
@@ -141,7 +189,7 @@ __device__ __forceinline__ double transfer5(const double* __restrict__ f, const 
     dm4 = dm4 * dm4 * dm4 * dm4;
     dm4 *= 0;
     const double gx = K4 * difu * difu * difu * difu + K3 * difu * difu * difu + K2 * difu * difu + K1 * difu + K0 - dm4;
-    const double fadd = t.wk * gx * f[(int64_t)iold * plane];
+    const double fadd = t.wk * gx * ldp(s.f, iold, plane, boff);
     if (finite_d(fadd)) fnew += fadd;
   }
   return fnew;
@@ -159,8 +207,8 @@ struct Lat<kD3Q27> {
   using Tgt = Tgt27;
   __device__ static double efc(int i, int d) { return (double)kE27[i][d]; }
   __device__ static Tgt tgt(int i, const double* g) { return make_tgt27(i, g[0], g[1], g[2], g[3]); }
-  __device__ static double xfer(const double* f, const double* m, int64_t plane, const Tgt& t, double sqTb) {
-    return transfer27(f, m, plane, t, sqTb);
+  __device__ static double xfer(const NodeSrc& s, uint32_t boff, const Tgt& t, double sqTb) {
+    return transfer27(s, boff, t, sqTb);
   }
 };
 
@@ -170,8 +218,8 @@ struct Lat<kD1Q5> {
   using Tgt = Tgt5;
   __device__ static double efc(int i, int d) { return d == 0 ? kA5[i] : 0.0; }
   __device__ static Tgt tgt(int i, const double* g) { return make_tgt5(i, g[0], g[3]); }
-  __device__ static double xfer(const double* f, const double* m, int64_t plane, const Tgt& t, double) {
-    return transfer5(f, m, plane, t);
+  __device__ static double xfer(const NodeSrc& s, uint32_t boff, const Tgt& t, double) {
+    return transfer5(s, boff, t);
   }
 };
 
@@ -194,12 +242,6 @@ __device__ __forceinline__ void lag4_factors(double s, double F[4][3]) {
       if (p != i) F[i][c++] = (s - p) / (double)(i - p);
   }
 }
-
-struct NodeSrc {
-  const double* __restrict__ f;
-  const double* __restrict__ m;
-  Grid g;
-};
 
 // propagateWENO<3,5> (steps.cu:774-825) with LagrEval<3> (steps.cu:924-937).
 // CENTRED=false: as written, y/z sub-stencil nodes floor(y+0.5)+0..2 and smoothness samples on
@@ -238,8 +280,7 @@ __device__ double interp_weno5(const NodeSrc& src, double cx, double cy, double 
     const int x = wrapi(r - 2 + xs, g.nx);
     double T[9];
     if (line) {
-      const int64_t c = x;
-      const double v = Lat<LAT>::xfer(src.f + c, src.m + c, g.plane, t, sqTb);
+      const double v = Lat<LAT>::xfer(src, (uint32_t)x * 8u, t, sqTb);
 #pragma unroll
       for (int q = 0; q < 9; q++) T[q] = v;
       ub[xs] = v;  // smoothness row (r mod 1, r mod 1) = (0,0): same node
@@ -248,14 +289,12 @@ __device__ double interp_weno5(const NodeSrc& src, double cx, double cy, double 
       for (int a = 0; a < 3; a++)
 #pragma unroll
         for (int b = 0; b < 3; b++) {
-          const int64_t c = cell_index(g, x, yy[a], zz[b]);
-          T[3 * a + b] = Lat<LAT>::xfer(src.f + c, src.m + c, g.plane, t, sqTb);
+          T[3 * a + b] = Lat<LAT>::xfer(src, node_off(g, x, yy[a], zz[b]), t, sqTb);
         }
       if (CENTRED) {
         ub[xs] = T[4];
       } else {
-        const int64_t cb = cell_index(g, x, by, bz);
-        ub[xs] = Lat<LAT>::xfer(src.f + cb, src.m + cb, g.plane, t, sqTb);
+        ub[xs] = Lat<LAT>::xfer(src, node_off(g, x, by, bz), t, sqTb);
       }
     }
     // accumulate into every sub-stencil that contains this x slice, in (iy, iz) order
@@ -322,7 +361,7 @@ __device__ double interp_lagr4(const NodeSrc& src, double cx, double cy, double 
     const int x = wrapi(lx + ix, g.nx);
     const double p = 1 * Fx[ix][0] * Fx[ix][1] * Fx[ix][2];
     double v1 = 0;
-    if (line) v1 = Lat<LAT>::xfer(src.f + x, src.m + x, g.plane, t, sqTb);
+    if (line) v1 = Lat<LAT>::xfer(src, (uint32_t)x * 8u, t, sqTb);
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
@@ -331,8 +370,7 @@ __device__ double interp_lagr4(const NodeSrc& src, double cx, double cy, double 
         if (line) {
           T = v1;
         } else {
-          const int64_t c = cell_index(g, x, yy[a], zz[b]);
-          T = Lat<LAT>::xfer(src.f + c, src.m + c, g.plane, t, sqTb);
+          T = Lat<LAT>::xfer(src, node_off(g, x, yy[a], zz[b]), t, sqTb);
         }
         fpr += p * Fy[a][0] * Fy[a][1] * Fy[a][2] * Fz[b][0] * Fz[b][1] * Fz[b][2] * T;
       }
@@ -344,9 +382,8 @@ __device__ double interp_lagr4(const NodeSrc& src, double cx, double cy, double 
 // One lane per cell: LBMpull (steps.cu:356-365, 421-512, 557-564, 582)
 
 template <int LAT, int INTERP>
-__global__ __launch_bounds__(64) void k_step_f64(const double* __restrict__ fin, const double* __restrict__ min,
-                                                 double* __restrict__ fout, double* __restrict__ mout,
-                                                 int* __restrict__ niter_out,
+__global__ __launch_bounds__(64) void k_step_f64(const NodeSrc src, double* __restrict__ fout,
+                                                 double* __restrict__ mout, int* __restrict__ niter_out,
                                                  const StepArgs a) {
   constexpr int Q = Lat<LAT>::Q;
   const Grid& g = a.g;
@@ -359,10 +396,9 @@ __global__ __launch_bounds__(64) void k_step_f64(const double* __restrict__ fin,
     const int z = (int)(cell / ((int64_t)g.nx * g.ny));
     const int64_t P = g.plane;
     // c0 = load; all-zero cell -> set_base(1, (0,0,0,TLat)) (steps.cu:360-363)
-    double rho0 = min[cell], c0[4] = {min[P + cell], min[2 * P + cell], min[3 * P + cell], min[4 * P + cell]};
+    double rho0 = src.m[cell], c0[4] = {src.m[P + cell], src.m[2 * P + cell], src.m[3 * P + cell], src.m[4 * P + cell]};
     const bool zero = rho0 == 0 && c0[0] == 0 && c0[1] == 0 && c0[2] == 0 && c0[3] == 0;
     if (zero) { rho0 = 1; c0[3] = a.tlat; }
-    const NodeSrc src{fin, min, g};
     while (niter < a.max_iter) {
       niter++;
       const double uT[4] = {c0[0], c0[1], c0[2], c0[3]};
@@ -384,7 +420,7 @@ __global__ __launch_bounds__(64) void k_step_f64(const double* __restrict__ fin,
         m2 += fn * vz;
         m3 += fn * (vx * vx + vy * vy + vz * vz);
         const int64_t o = (int64_t)i * P + cell;
-        const double fo = niter > 1 ? fout[o] : (zero ? a.w[i] * 1.0 : fin[o]);
+        const double fo = niter > 1 ? fout[o] : (zero ? a.w[i] * 1.0 : src.f[o]);
         if (fabs(fn - fo) >= a.tol_abs + a.tol_rel * fabs(fn)) conv = false;
         fout[o] = fn;
       }
@@ -421,14 +457,15 @@ __global__ __launch_bounds__(64) void k_step_f64(const double* __restrict__ fin,
 // D3Q125: one block per cell, lane = target population (PonDstepD3Q125, steps.cu:151-250)
 
 __global__ __launch_bounds__(128) void k_step_q125_f64(const double* __restrict__ fin, const double* __restrict__ min,
+                                                       const unsigned char* __restrict__ okin,
                                                        double* __restrict__ fout, double* __restrict__ mout,
-                                                       int* __restrict__ niter_out,
-                                                       const StepArgs a) {
+                                                       int* __restrict__ niter_out, const StepArgs a) {
   constexpr int Q = 125;
   __shared__ double sf[27][Q];   // fixed neighbourhood populations (cfix, steps.cu:150)
   __shared__ double sg[27][4];   // their gauges
   __shared__ double snew[Q];     // cnew.f
   __shared__ int sflag[2];
+  __shared__ int sok[27];        // node populations finite and below 2^200
   const Grid& g = a.g;
   const int64_t P = g.plane;
   const int64_t cell = blockIdx.x;
@@ -442,6 +479,7 @@ __global__ __launch_bounds__(128) void k_step_q125_f64(const double* __restrict_
     const int64_t c = cell_index(g, px, py, pz);
     if (lane_ok) sf[n][k] = fin[(int64_t)k * P + c];
     if (k < 4) sg[n][k] = min[(int64_t)(k + 1) * P + c];
+    if (k == 4) sok[n] = okin[c];
   }
   double rho0 = min[cell], c0[4] = {min[P + cell], min[2 * P + cell], min[3 * P + cell], min[4 * P + cell]};
   const bool zero = rho0 == 0 && c0[0] == 0 && c0[1] == 0 && c0[2] == 0 && c0[3] == 0;
@@ -469,6 +507,7 @@ __global__ __launch_bounds__(128) void k_step_q125_f64(const double* __restrict_
       else wv[d] = 1. / (2 * e2_to * e2_to * (e2_to * e2_to - e1_to * e1_to));
     }
     const double W = wv[0] * wv[1] * wv[2];
+    const bool wok = fabs(W) < kSafe;
     const double vk[3] = {s * kA5[X[0]] + uT[0], s * kA5[X[1]] + uT[1], s * kA5[X[2]] + uT[2]};
     const double sh[3] = {((double)x - vk[0]) - (double)(x - 1), ((double)y - vk[1]) - (double)(y - 1),
                           ((double)z - vk[2]) - (double)(z - 1)};
@@ -491,6 +530,7 @@ __global__ __launch_bounds__(128) void k_step_q125_f64(const double* __restrict_
     for (int n = 0; n < 27; n++) {
       const double S0 = sqrts_d(sg[n][3] / 1.0);
       double gb[3][5];
+      bool safe = wok && sok[n] != 0;
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         const double du = sg[n][d] - uT[d];
@@ -498,20 +538,33 @@ __global__ __launch_bounds__(128) void k_step_q125_f64(const double* __restrict_
         for (int j = 0; j < 5; j++) {
           const double b = du + kA5[j] * S0;
           gb[d][j] = (b + off[d][0]) * (b + off[d][1]) * (b + off[d][2]) * (b + off[d][3]);
+          safe = safe && fabs(gb[d][j]) < kSafe;
         }
       }
       double fnew = 0;
+      if (safe) {
+        // every term is finite: |W|,|gx|,|gy|,|gz|,|f| < 2^200
 #pragma unroll
-      for (int jx = 0; jx < 5; jx++)
+        for (int jx = 0; jx < 5; jx++)
 #pragma unroll
-        for (int jy = 0; jy < 5; jy++) {
-          const double gxy = gb[0][jx] * gb[1][jy];
+          for (int jy = 0; jy < 5; jy++) {
+            const double gxy = gb[0][jx] * gb[1][jy];
 #pragma unroll
-          for (int jz = 0; jz < 5; jz++) {
-            const double fadd = W * (gxy * gb[2][jz]) * sf[n][25 * jx + 5 * jy + jz];
-            if (finite_d(fadd)) fnew += fadd;
+            for (int jz = 0; jz < 5; jz++) fnew += W * (gxy * gb[2][jz]) * sf[n][25 * jx + 5 * jy + jz];
           }
-        }
+      } else {
+#pragma unroll
+        for (int jx = 0; jx < 5; jx++)
+#pragma unroll
+          for (int jy = 0; jy < 5; jy++) {
+            const double gxy = gb[0][jx] * gb[1][jy];
+#pragma unroll
+            for (int jz = 0; jz < 5; jz++) {
+              const double fadd = W * (gxy * gb[2][jz]) * sf[n][25 * jx + 5 * jy + jz];
+              if (finite_d(fadd)) fnew += fadd;
+            }
+          }
+      }
       const int ia = n / 9, ib = (n / 3) % 3, ic = n % 3;
       fpr += 1 * Fx[ia][0] * Fx[ia][1] * Fy[ib][0] * Fy[ib][1] * Fz[ic][0] * Fz[ic][1] * fnew;
     }
@@ -567,16 +620,43 @@ __global__ __launch_bounds__(128) void k_step_q125_f64(const double* __restrict_
 // ---------------------------------------------------------------------------------------------
 // host-side launchers
 
-hipError_t launch_step_f64(int lattice, int interp, const double* fin, const double* min, double* fout, double* mout,
-                           int* niter, const StepArgs& a, hipStream_t st) {
+// per-step node data: sqrt(|T|)*sign(T) and the "populations finite and below 2^200" flag
+template <int Q>
+__global__ void k_node_prep(const double* __restrict__ f, const double* __restrict__ m, double* __restrict__ P,
+                            unsigned char* __restrict__ ok, int64_t n, int64_t plane) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const double T = m[4 * plane + c];
+  P[c] = sqrt(fabs(T)) * (double)isign_d(T);
+  bool good = true;
+  for (int i = 0; i < Q; i++) good = good && fabs(f[(int64_t)i * plane + c]) < kSafe;
+  ok[c] = good ? 1 : 0;
+}
+
+hipError_t launch_node_prep(int lattice, const double* f, const double* m, double* P, unsigned char* ok, int64_t n,
+                            int64_t plane, hipStream_t st) {
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  if (lattice == kD3Q27)
+    hipLaunchKernelGGL(k_node_prep<27>, dim3(blocks), dim3(256), 0, st, f, m, P, ok, n, plane);
+  else if (lattice == kD1Q5)
+    hipLaunchKernelGGL(k_node_prep<5>, dim3(blocks), dim3(256), 0, st, f, m, P, ok, n, plane);
+  else
+    hipLaunchKernelGGL(k_node_prep<125>, dim3(blocks), dim3(256), 0, st, f, m, P, ok, n, plane);
+  return hipGetLastError();
+}
+
+hipError_t launch_step_f64(int lattice, int interp, const double* fin, const double* min, const double* Pin,
+                           const unsigned char* okin, double* fout, double* mout, int* niter, const StepArgs& a,
+                           hipStream_t st) {
   if (lattice == kD3Q125) {
     const int64_t blocks = a.g.n;
-    hipLaunchKernelGGL(k_step_q125_f64, dim3((unsigned)blocks), dim3(128), 0, st, fin, min, fout, mout, niter, a);
+    hipLaunchKernelGGL(k_step_q125_f64, dim3((unsigned)blocks), dim3(128), 0, st, fin, min, okin, fout, mout, niter,
+                       a);
     return hipGetLastError();
   }
   const unsigned blocks = (unsigned)((a.g.n + 63) / 64);
-#define POND_LAUNCH(L, I)                                                                                  \
-  hipLaunchKernelGGL((k_step_f64<L, I>), dim3(blocks), dim3(64), 0, st, fin, min, fout, mout, niter, a)
+  const NodeSrc src{fin, min, Pin, okin, a.g};
+#define POND_LAUNCH(L, I) hipLaunchKernelGGL((k_step_f64<L, I>), dim3(blocks), dim3(64), 0, st, src, fout, mout, niter, a)
   if (lattice == kD3Q27) {
     if (interp == 0) POND_LAUNCH(kD3Q27, 0);
     else if (interp == 1) POND_LAUNCH(kD3Q27, 1);
