@@ -34,7 +34,6 @@ import numpy as np  # noqa: E402
 METRIC = "MLUPS + mean gauge iters/step, learned+FP32 vs plain FP64, at stated L∞ err"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-FIELDS = ("rho", "ux", "uy", "uz", "T")
 
 # algorithmic bytes per lattice update: read one state, write one state, int32 iterations
 BYTES_PER_LUP = {("D3Q27", "fp64"): 2 * (27 * 8 + 5 * 8) + 4, ("D3Q27", "split"): 2 * (27 * 4 + 4 * 4 + 8) + 4,
@@ -69,6 +68,15 @@ def interp_for(case):
 def rel_linf(a, b):
     den = np.abs(b).max()
     return float(np.abs(a - b).max() / den) if den > 0 else float(np.abs(a - b).max())
+
+
+def state_errors(got, ref):
+    """relative L-inf of rho and T per field and of u as a vector (max component error over the
+    largest reference velocity component)"""
+    du = max(np.abs(got[k] - ref[k]).max() for k in ("ux", "uy", "uz"))
+    um = max(np.abs(ref[k]).max() for k in ("ux", "uy", "uz"))
+    return {"rho": rel_linf(got["rho"], ref["rho"]), "u": float(du / um) if um > 0 else float(du),
+            "T": rel_linf(got["T"], ref["T"])}
 
 
 def run_path(case, precision, learned, collect, warmup, steps, train_cfg):
@@ -134,6 +142,27 @@ def cpu_baseline(case_name, interp):
                       f"({cells} cell-steps in {dt:.1f} s, mean gauge iterations {niter:.2f})"}
 
 
+def copy_calibration(nbytes):
+    """One device-to-device copy of nbytes through the HIP runtime: the reference point that
+    scales rocprof's FETCH_SIZE / WRITE_SIZE readings (a plain copy reads and writes nbytes)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(nbytes)) != 0:
+            raise SystemExit("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    t = time.perf_counter()
+    if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+        raise SystemExit("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    dt = time.perf_counter() - t
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": nbytes, "seconds": dt, "GBps_rw": 2 * nbytes / dt / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,7 +173,12 @@ def main():
     ap.add_argument("--n", type=int, default=None, help="grid edge override (testing only)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--paths", default="fp64,split,learned")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run one device copy of BYTES (rocprof traffic calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; how the step would shard is described in DESIGN.md; "
               "use --gpus 1", file=sys.stderr)
@@ -168,7 +202,7 @@ def main():
              "ms_per_step": round(r["wall_s"] * 1e3 / args.steps, 3),
              "kernel_ms_per_step": round(r["kernel_ms"] / args.steps, 3)}
         if ref is not None and name != "fp64":
-            s["rel_linf_vs_fp64"] = {k: float(f"{rel_linf(r['fields'][k], ref['fields'][k]):.3e}") for k in FIELDS}
+            s["rel_linf_vs_fp64"] = {k: float(f"{v:.3e}") for k, v in state_errors(r["fields"], ref["fields"]).items()}
         for k in ("train_s", "train_samples", "train_loss"):
             if k in r:
                 s[k] = r[k]

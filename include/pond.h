@@ -86,10 +86,12 @@ typedef struct {
 } pond_diag;
 
 typedef struct {
-  double last_ms;    /* device time of the last pond_step call (HIP events on the ctx stream) */
-  double total_ms;   /* accumulated device time of all pond_step calls */
+  double last_ms;    /* device time of the step kernel(s) of the last pond_step call (HIP events
+                        on the ctx stream around each step kernel launch) */
+  double total_ms;   /* accumulated step-kernel time of all pond_step calls */
   int64_t steps;     /* steps covered by total_ms */
-  double last_mlp_ms;/* part of last_ms spent in the learned initializer */
+  double last_mlp_ms;/* device time of the learned initializer's inference in the last step
+                        (outside last_ms) */
 } pond_timing;
 
 typedef struct pond_ctx pond_ctx;

@@ -46,7 +46,8 @@ hipError_t mlp_to_bf16(const float* src, int rows, int cols, int src_ld, unsigne
 hipError_t mlp_adam(float* p, const float* g, float* m, float* v, int n, float lr, float b1, float b2, float eps,
                     float c1, float c2, hipStream_t st);
 hipError_t mlp_gather_batch(const float* Xall, const float* Yall, const int* idx, int B, const float* in_inv,
-                            const float* out_inv, float* X, float* T, hipStream_t st);
+                            const float* out_inv, uint64_t seed, int augment, float rel_eps, float* X, float* T,
+                            hipStream_t st);
 hipError_t mlp_collect(const MlpState& prev, const MlpState& next, int64_t nsamp, uint64_t seed, float* X, float* Y,
                        hipStream_t st);
 hipError_t mlp_moments2(const float* X, const float* Y, int64_t n, double* acc, hipStream_t st);

@@ -292,6 +292,9 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
                        int n, float lr, float b1, float b2, float eps, float c1, float c2) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
+  // the biases stay where they are (zero after pond_mlp_init): a bias-free ReLU network is
+  // positively homogeneous, net(a x) = a net(x) for a >= 0
+  if ((t >= kOffB1 && t < kOffW2) || (t >= kOffB2 && t < kOffW3) || t >= kOffB3) return;
   const float gi = g[t];
   const float mi = b1 * m[t] + (1.f - b1) * gi;
   const float vi = b2 * v[t] + (1.f - b2) * gi * gi;
@@ -301,15 +304,58 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 }
 
 // gather a minibatch: X (normalized features, B x kMlpIn) and T (normalized targets) from the
-// dataset rows idx[b]
+// dataset rows idx[b].
+//  * Augmentation: row b is seen through one of the 48 symmetries of the cubic lattice (an
+//    axis permutation with sign flips, chosen by a hash of (seed, b)).  New axis a is old axis
+//    perm[a] times sign[a]: the neighbour at new offset d' is the old neighbour at
+//    d_perm[a] = sign[a] d'_a, and velocity component a is sign[a] times old component perm[a].
+//  * Relative error: the pair is scaled by 1 / sqrt(max_k T_k^2 + rel_eps^2).  The network is
+//    positively homogeneous, so this turns the squared error into a relative squared error and
+//    leaves the prediction for the unscaled features unchanged.
 __global__ void k_gather_batch(const float* __restrict__ Xall, const float* __restrict__ Yall,
                                const int* __restrict__ idx, int B, const float* __restrict__ in_inv,
-                               const float* __restrict__ out_inv, float* __restrict__ X, float* __restrict__ T) {
+                               const float* __restrict__ out_inv, uint64_t seed, int augment, float rel_eps,
+                               float* __restrict__ X, float* __restrict__ T) {
   const int b = blockIdx.x;
   if (b >= B) return;
   const int64_t r = idx[b];
-  for (int f = threadIdx.x; f < kMlpIn; f += blockDim.x) X[(int64_t)b * kMlpIn + f] = Xall[r * kMlpIn + f] * in_inv[f % 5];
-  if (threadIdx.x < kMlpOut) T[(int64_t)b * kMlpOut + threadIdx.x] = Yall[r * kMlpOut + threadIdx.x] * out_inv[threadIdx.x];
+  int perm[3] = {0, 1, 2}, sgn[3] = {1, 1, 1};
+  if (augment) {
+    uint64_t h = seed + 0x9e3779b97f4a7c15ull * (uint64_t)(b + 1);
+    h = (h ^ (h >> 30)) * 0xbf58476d1ce4e5b9ull;
+    h = (h ^ (h >> 27)) * 0x94d049bb133111ebull;
+    h ^= h >> 31;
+    const int sym = (int)(h % 48u);
+    const int pi = sym >> 3;  // 6 permutations
+    constexpr int kPerm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    for (int a = 0; a < 3; a++) {
+      perm[a] = kPerm[pi][a];
+      sgn[a] = (sym >> a) & 1 ? -1 : 1;
+    }
+  }
+  float t[kMlpOut];
+  float tmax = 0.f;
+  for (int c = 0; c < kMlpOut; c++) {
+    const int src = c < 3 ? perm[c] : c;
+    t[c] = Yall[r * kMlpOut + src] * out_inv[c] * (c < 3 ? (float)sgn[c] : 1.f);
+    tmax = fmaxf(tmax, fabsf(t[c]));
+  }
+  const float sc = 1.0f / sqrtf(tmax * tmax + rel_eps * rel_eps);
+  for (int f = threadIdx.x; f < kMlpIn; f += blockDim.x) {
+    const int nb = f / 5, type = f - 5 * nb;
+    const int dn[3] = {nb / 9 - 1, (nb / 3) % 3 - 1, nb % 3 - 1};
+    int d[3];
+    for (int a = 0; a < 3; a++) d[perm[a]] = sgn[a] * dn[a];
+    const int onb = 9 * (d[0] + 1) + 3 * (d[1] + 1) + (d[2] + 1);
+    float sgv = 1.f;
+    int otype = type;
+    if (type >= 1 && type <= 3) {
+      otype = 1 + perm[type - 1];
+      sgv = (float)sgn[type - 1];
+    }
+    X[(int64_t)b * kMlpIn + f] = Xall[r * kMlpIn + 5 * onb + otype] * sgv * in_inv[type] * sc;
+  }
+  if (threadIdx.x < kMlpOut) T[(int64_t)b * kMlpOut + threadIdx.x] = t[threadIdx.x] * sc;
 }
 
 // dataset collection: raw features of sampled cells from the previous state, raw targets
@@ -427,8 +473,10 @@ hipError_t mlp_adam(float* p, const float* g, float* m, float* v, int n, float l
 }
 
 hipError_t mlp_gather_batch(const float* Xall, const float* Yall, const int* idx, int B, const float* in_inv,
-                            const float* out_inv, float* X, float* T, hipStream_t st) {
-  hipLaunchKernelGGL(k_gather_batch, dim3(B), dim3(64), 0, st, Xall, Yall, idx, B, in_inv, out_inv, X, T);
+                            const float* out_inv, uint64_t seed, int augment, float rel_eps, float* X, float* T,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_batch, dim3(B), dim3(64), 0, st, Xall, Yall, idx, B, in_inv, out_inv, seed, augment,
+                     rel_eps, X, T);
   return hipGetLastError();
 }
 

@@ -12,6 +12,10 @@ using namespace pond;
 
 namespace {
 
+// relative-error floor of the training loss, in units of the RMS target (k_gather_batch)
+constexpr float kRelEps = 0.1f;
+constexpr double kPi = 3.14159265358979323846;
+
 #define MLP_HIP(ctx, expr)                                                        \
   do {                                                                            \
     hipError_t e_ = (expr);                                                       \
@@ -165,7 +169,7 @@ pond_status pond_mlp_init(pond_ctx* ctx, uint64_t seed) {
   if (!ctx) return POND_ERR_INVALID;
   pond_status s = mlp_alloc(ctx);
   if (s != POND_OK) return s;
-  // He-uniform weights, zero biases, from a seeded generator
+  // He-uniform weights and zero biases from a seeded generator; training keeps the biases at zero
   std::vector<float> p(kMlpParams, 0.0f);
   std::mt19937_64 rng(seed);
   auto fill = [&](int off, int fan_in, int n) {
@@ -242,17 +246,18 @@ pond_status pond_mlp_train(pond_ctx* ctx, int epochs, int batch, float lr, uint6
   MLP_HIP(ctx, mlp_moments2(m.X, m.Y, m.count, m.mom, ctx->stream));
   double mom[16];
   MLP_HIP(ctx, hipMemcpy(mom, m.mom, 16 * sizeof(double), hipMemcpyDeviceToHost));
-  for (int k = 0; k < 5; k++) {
-    const double ms = mom[k] / ((double)m.count * 27.0);
-    m.in_inv[k] = ms > 0 ? (float)(1.0 / std::sqrt(ms)) : 1.0f;
-  }
+  // the three velocity components share one scale (the lattice symmetries used in training
+  // exchange them)
+  double msi[5], mso[4];
+  for (int k = 0; k < 5; k++) msi[k] = mom[k] / ((double)m.count * 27.0);
+  for (int o = 0; o < 4; o++) mso[o] = mom[5 + o] / (double)m.count;
+  msi[1] = msi[2] = msi[3] = (msi[1] + msi[2] + msi[3]) / 3.0;
+  mso[0] = mso[1] = mso[2] = (mso[0] + mso[1] + mso[2]) / 3.0;
+  for (int k = 0; k < 5; k++) m.in_inv[k] = msi[k] > 0 ? (float)(1.0 / std::sqrt(msi[k])) : 1.0f;
   for (int o = 0; o < 4; o++) {
-    const double ms = mom[5 + o] / (double)m.count;
-    m.out_scale[o] = ms > 0 ? (float)std::sqrt(ms) : 1.0f;
+    m.out_scale[o] = mso[o] > 0 ? (float)std::sqrt(mso[o]) : 1.0f;
     m.out_inv[o] = 1.0f / m.out_scale[o];
   }
-  float* d_in_inv = m.bx;  // scales live in the batch buffers' tail during gather: use dedicated copies
-  (void)d_in_inv;
   float *din = nullptr, *dout = nullptr;
   MLP_HIP(ctx, hipMalloc(&din, 5 * sizeof(float)));
   MLP_HIP(ctx, hipMalloc(&dout, 4 * sizeof(float)));
@@ -263,6 +268,7 @@ pond_status pond_mlp_train(pond_ctx* ctx, int epochs, int batch, float lr, uint6
   for (int64_t i = 0; i < m.count; i++) perm[i] = (int)i;
   const int64_t nb = m.count / batch;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const double total = (double)epochs * (double)nb;  // cosine decay of the step size over the run
   pond_status st = POND_OK;
   for (int ep = 0; ep < epochs && st == POND_OK; ep++) {
     std::shuffle(perm.begin(), perm.end(), rng);
@@ -271,7 +277,8 @@ pond_status pond_mlp_train(pond_ctx* ctx, int epochs, int batch, float lr, uint6
     for (int64_t bi = 0; bi < nb && st == POND_OK; bi++) {
       if (hipMemcpyAsync(m.idx, perm.data() + bi * batch, (size_t)batch * sizeof(int), hipMemcpyHostToDevice,
                          ctx->stream) != hipSuccess ||
-          mlp_gather_batch(m.X, m.Y, m.idx, batch, din, dout, m.bx, m.bt, ctx->stream) != hipSuccess) {
+          mlp_gather_batch(m.X, m.Y, m.idx, batch, din, dout, seed + 0x2545f4914f6cdd1dull * (uint64_t)(ep * nb + bi + 1),
+                           1, kRelEps, m.bx, m.bt, ctx->stream) != hipSuccess) {
         st = POND_ERR_HIP;
         ctx->err = "pond_mlp_train: batch gather failed";
         break;
@@ -280,7 +287,8 @@ pond_status pond_mlp_train(pond_ctx* ctx, int epochs, int batch, float lr, uint6
       if (st != POND_OK) break;
       m.adam_t++;
       const float c1 = 1.0f - std::pow(b1, (float)m.adam_t), c2 = 1.0f - std::pow(b2, (float)m.adam_t);
-      if (mlp_adam(m.params, m.grad, m.adam_m, m.adam_v, kMlpParams, lr, b1, b2, eps, c1, c2, ctx->stream) !=
+      const float lr_t = (float)(0.5 * (double)lr * (1.0 + std::cos(kPi * (double)(ep * nb + bi) / total)));
+      if (mlp_adam(m.params, m.grad, m.adam_m, m.adam_v, kMlpParams, lr_t, b1, b2, eps, c1, c2, ctx->stream) !=
           hipSuccess) {
         st = POND_ERR_HIP;
         ctx->err = "pond_mlp_train: adam failed";

@@ -460,10 +460,9 @@ pond_status pond_step(pond_ctx* ctx, int nsteps) {
       he = launch_sqrt_t(ctx->sp[in].T, ctx->sq, ctx->g.n, ctx->stream);
       if (he != hipSuccess) return fail(ctx, POND_ERR_HIP, std::string("sqrt(T) kernel: ") + hipGetErrorString(he));
     }
-    POND_HIP(ctx, hipEventRecord(e.a, ctx->stream));
     const bool learned = ctx->split && ctx->p.use_mlp && ctx->mlp.trained;
     if (learned) {
-      // predicted starting gauge from the load state (part of the step's timed work)
+      // predicted starting gauge from the load state (part of the step's work)
       POND_HIP(ctx, hipEventRecord(ctx->mlp_ev[0], ctx->stream));
       he = mlp_launch_infer(pond_mlp_weights(ctx), MlpState{ctx->g, ctx->sp[in].m, ctx->sp[in].T}, ctx->guess,
                             ctx->g.plane, ctx->stream);
@@ -471,17 +470,21 @@ pond_status pond_step(pond_ctx* ctx, int nsteps) {
       POND_HIP(ctx, hipEventRecord(ctx->mlp_ev[1], ctx->stream));
       ctx->mlp_pending = true;
     }
+    if (!ctx->split) {
+      he = launch_node_prep(ctx->p.lattice, ctx->f64[in].f, ctx->f64[in].m, ctx->nodeP, ctx->nodeok, ctx->g.n,
+                            ctx->g.plane, ctx->stream);
+      if (he != hipSuccess) return fail(ctx, POND_ERR_HIP, std::string("node prep kernel: ") + hipGetErrorString(he));
+    }
+    // the event pair brackets the step kernel alone
+    POND_HIP(ctx, hipEventRecord(e.a, ctx->stream));
     if (ctx->split) {
       const SplitPtrs pin{ctx->sp[in].f, ctx->sp[in].m, ctx->sp[in].T};
       const SplitPtrs pout{ctx->sp[outb].f, ctx->sp[outb].m, ctx->sp[outb].T};
       he = launch_step_split(ctx->p.lattice, ctx->p.interp, pin, pout, ctx->sq, ctx->niter,
                              learned ? ctx->guess : nullptr, ctx->sargs, ctx->stream);
     } else {
-      he = launch_node_prep(ctx->p.lattice, ctx->f64[in].f, ctx->f64[in].m, ctx->nodeP, ctx->nodeok, ctx->g.n,
-                            ctx->g.plane, ctx->stream);
-      if (he == hipSuccess)
-        he = launch_step_f64(ctx->p.lattice, ctx->p.interp, ctx->f64[in].f, ctx->f64[in].m, ctx->nodeP,
-                             ctx->nodeok, ctx->f64[outb].f, ctx->f64[outb].m, ctx->niter, ctx->args, ctx->stream);
+      he = launch_step_f64(ctx->p.lattice, ctx->p.interp, ctx->f64[in].f, ctx->f64[in].m, ctx->nodeP, ctx->nodeok,
+                           ctx->f64[outb].f, ctx->f64[outb].m, ctx->niter, ctx->args, ctx->stream);
     }
     if (he != hipSuccess) return fail(ctx, POND_ERR_HIP, std::string("step kernel launch: ") + hipGetErrorString(he));
     POND_HIP(ctx, hipEventRecord(e.b, ctx->stream));

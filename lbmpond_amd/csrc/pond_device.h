@@ -26,13 +26,6 @@ struct StepArgs {
   double w[kQmax];    // lattice weights for the collide and set_base
 };
 
-// per-step counters written by the step kernels (one atomic per wave)
-struct StepStats {
-  unsigned long long niter_sum;
-  int niter_max;
-  int pad;
-};
-
 __device__ __forceinline__ int isign_d(double v) { return (0.0 < v) - (v < 0.0); }
 
 // params.h:110-112
@@ -62,20 +55,6 @@ __device__ __forceinline__ int floor_int(double v) {
 
 __device__ __forceinline__ int64_t cell_index(const Grid& g, int x, int y, int z) {
   return (int64_t)x + (int64_t)g.nx * ((int64_t)y + (int64_t)g.ny * (int64_t)z);
-}
-
-// wave-level reductions for the per-step counters
-__device__ __forceinline__ void stats_add(StepStats* st, int niter, bool active) {
-  unsigned long long v = active ? (unsigned long long)niter : 0ull;
-  int m = active ? niter : 0;
-  for (int off = 32; off > 0; off >>= 1) {
-    v += __shfl_xor(v, off, 64);
-    m = max(m, __shfl_xor(m, off, 64));
-  }
-  if ((threadIdx.x & 63) == 0) {
-    atomicAdd(&st->niter_sum, v);
-    atomicMax(&st->niter_max, m);
-  }
 }
 
 }  // namespace pond

@@ -17,6 +17,20 @@ def rel_linf(a, b):
     return float(num / den) if den > 0 else float(num)
 
 
+STATE_ERRORS = ("rho", "u", "T")
+
+
+def state_errors(got, ref):
+    """Relative L-inf errors of density, velocity and temperature: rho and T per field, the
+    velocity as a vector, max_k |u_k - u_k,ref| / max_k |u_k,ref| over all cells, so that a
+    component that is zero in the reference (the axial velocity of a vortex tube) is measured
+    against the flow speed rather than against its own round-off."""
+    du = max(np.abs(np.asarray(got[k], np.float64) - np.asarray(ref[k], np.float64)).max() for k in ("ux", "uy", "uz"))
+    um = max(np.abs(np.asarray(ref[k], np.float64)).max() for k in ("ux", "uy", "uz"))
+    return {"rho": rel_linf(got["rho"], ref["rho"]), "u": float(du / um) if um > 0 else float(du),
+            "T": rel_linf(got["T"], ref["T"])}
+
+
 def smooth_random_state(shape, seed, T0=0.02, u_amp=0.05, rho_amp=0.1, dims=3):
     """Seeded smooth random (rho, u, T): a few random Fourier modes per field."""
     rng = np.random.default_rng(seed)
