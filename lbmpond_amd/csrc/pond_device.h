@@ -53,6 +53,17 @@ __device__ __forceinline__ int floor_int(double v) {
   return (int)f;
 }
 
+// Workgroups are handed to the 8 XCDs of the chip round-robin (workgroup b runs on XCD b % 8),
+// and each XCD has its own L2.  xcd_block maps the hardware order to a logical block order in
+// which XCD k works through one contiguous slab of blocks, so the stencil neighbourhoods of the
+// workgroups resident on one XCD overlap in its L2.  A bijection on [0, nblocks) for any count.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks) {
+  constexpr int kXcd = 8;
+  const int64_t q = nblocks / kXcd, r = nblocks % kXcd;
+  const int64_t x = b % kXcd, local = b / kXcd;
+  return x * q + (x < r ? x : r) + local;
+}
+
 __device__ __forceinline__ int64_t cell_index(const Grid& g, int x, int y, int z) {
   return (int64_t)x + (int64_t)g.nx * ((int64_t)y + (int64_t)g.ny * (int64_t)z);
 }

@@ -387,7 +387,7 @@ __global__ __launch_bounds__(64) void k_step_f64(const NodeSrc src, double* __re
                                                  const StepArgs a) {
   constexpr int Q = Lat<LAT>::Q;
   const Grid& g = a.g;
-  const int64_t cell = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  const int64_t cell = xcd_block(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
   const bool active = cell < g.n;
   int niter = 0;
   if (active) {
@@ -468,7 +468,7 @@ __global__ __launch_bounds__(128) void k_step_q125_f64(const double* __restrict_
   __shared__ int sok[27];        // node populations finite and below 2^200
   const Grid& g = a.g;
   const int64_t P = g.plane;
-  const int64_t cell = blockIdx.x;
+  const int64_t cell = xcd_block(blockIdx.x, gridDim.x);
   const int k = threadIdx.x;
   const bool lane_ok = k < Q;
   const int x = (int)(cell % g.nx);
