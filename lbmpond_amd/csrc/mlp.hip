@@ -263,19 +263,35 @@ __global__ void k_relu_grad(const float* __restrict__ G, const float* __restrict
 }
 
 // column sums of a rows x cols matrix (row stride ld) into out[cols] (bias gradients)
-__global__ void k_colsum(const float* __restrict__ M, int rows, int cols, int ld, float* __restrict__ out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+// column sums of a rows x cols matrix (stride ld): one 256-thread block per column
+__global__ __launch_bounds__(256) void k_colsum(const float* __restrict__ M, int rows, int cols, int ld,
+                                                float* __restrict__ out) {
+  __shared__ float sh[256];
+  const int c = blockIdx.x;
   if (c >= cols) return;
   float s = 0.f;
-  for (int r = 0; r < rows; r++) s += M[(int64_t)r * ld + c];
-  out[c] = s;
+  for (int r = threadIdx.x; r < rows; r += 256) s += M[(int64_t)r * ld + c];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = sh[0];
 }
 
-__global__ void k_sum(const float* __restrict__ v, int n, float* __restrict__ out) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// sum of n values, one 256-thread block
+__global__ __launch_bounds__(256) void k_sum(const float* __restrict__ v, int n, float* __restrict__ out) {
+  __shared__ float sh[256];
   float s = 0.f;
-  for (int i = 0; i < n; i++) s += v[i];
-  *out = s;
+  for (int i = threadIdx.x; i < n; i += 256) s += v[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
 }
 
 // FP32 -> bf16 with row padding: src rows x cols (stride src_ld) -> dst rows x ld
@@ -452,12 +468,12 @@ hipError_t mlp_relu_grad(const float* G, const float* Z, float* dZ, unsigned sho
 }
 
 hipError_t mlp_colsum(const float* M, int rows, int cols, int ld, float* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_colsum, dim3(nb(cols, 64)), dim3(64), 0, st, M, rows, cols, ld, out);
+  hipLaunchKernelGGL(k_colsum, dim3((unsigned)cols), dim3(256), 0, st, M, rows, cols, ld, out);
   return hipGetLastError();
 }
 
 hipError_t mlp_sum(const float* v, int n, float* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_sum, dim3(1), dim3(64), 0, st, v, n, out);
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, st, v, n, out);
   return hipGetLastError();
 }
 

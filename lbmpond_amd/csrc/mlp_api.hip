@@ -99,8 +99,9 @@ pond_status mlp_refresh_bf16(pond_ctx* ctx) {
 }
 
 // forward + backward of one minibatch held in m.bx (normalized X) / m.bt (normalized T);
-// gradients land in m.grad, the summed squared error in m.loss[0]
-pond_status mlp_fwd_bwd(pond_ctx* ctx, int B) {
+// gradients land in m.grad, the summed squared error in m.loss[0].  Training holds the biases
+// at zero and skips their gradients (bias_grad = false); the gradient entry point computes all.
+pond_status mlp_fwd_bwd(pond_ctx* ctx, int B, bool bias_grad) {
   MlpCtx& m = ctx->mlp;
   hipStream_t st = ctx->stream;
   const float* P = m.params;
@@ -115,18 +116,18 @@ pond_status mlp_fwd_bwd(pond_ctx* ctx, int B) {
   MLP_HIP(ctx, mlp_bias_relu(m.yacc, P + kOffB3, m.y, m.dyb, B, kMlpOut, kMlpOutPad, 0, st));
   // backward
   MLP_HIP(ctx, mlp_out_grad(m.y, m.bt, m.dy, m.dyb, m.sqerr, B, kMlpOut, kMlpOutPad, st));
-  MLP_HIP(ctx, mlp_colsum(m.dy, B, kMlpOut, kMlpOutPad, m.grad + kOffB3, st));
+  if (bias_grad) MLP_HIP(ctx, mlp_colsum(m.dy, B, kMlpOut, kMlpOutPad, m.grad + kOffB3, st));
   MLP_HIP(ctx, mlp_gemm(m.a2b, 1, kMlpH, m.dyb, kMlpOutPad, 1, m.gw3tmp, kMlpOutPad, kMlpH, kMlpOutPad, B, 0, st));
   // pack the 64x16 padded W3 gradient into the 64x4 parameter block
   MLP_HIP(ctx, hipMemcpy2DAsync(m.grad + kOffW3, kMlpOut * sizeof(float), m.gw3tmp, kMlpOutPad * sizeof(float),
                                 kMlpOut * sizeof(float), kMlpH, hipMemcpyDeviceToDevice, st));
   MLP_HIP(ctx, mlp_gemm(m.dyb, kMlpOutPad, 1, m.W3b, 1, kMlpOutPad, m.g2, kMlpH, B, kMlpH, kMlpOutPad, 0, st));
   MLP_HIP(ctx, mlp_relu_grad(m.g2, m.z2, m.dz2, m.dz2b, (int64_t)B * kMlpH, st));
-  MLP_HIP(ctx, mlp_colsum(m.dz2, B, kMlpH, kMlpH, m.grad + kOffB2, st));
+  if (bias_grad) MLP_HIP(ctx, mlp_colsum(m.dz2, B, kMlpH, kMlpH, m.grad + kOffB2, st));
   MLP_HIP(ctx, mlp_gemm(m.a1b, 1, kMlpH, m.dz2b, kMlpH, 1, m.grad + kOffW2, kMlpH, kMlpH, kMlpH, B, 0, st));
   MLP_HIP(ctx, mlp_gemm(m.dz2b, kMlpH, 1, m.W2b, 1, kMlpH, m.g1, kMlpH, B, kMlpH, kMlpH, 0, st));
   MLP_HIP(ctx, mlp_relu_grad(m.g1, m.z1, m.dz1, m.dz1b, (int64_t)B * kMlpH, st));
-  MLP_HIP(ctx, mlp_colsum(m.dz1, B, kMlpH, kMlpH, m.grad + kOffB1, st));
+  if (bias_grad) MLP_HIP(ctx, mlp_colsum(m.dz1, B, kMlpH, kMlpH, m.grad + kOffB1, st));
   MLP_HIP(ctx, mlp_gemm(m.xb, 1, kMlpInPad, m.dz1b, kMlpH, 1, m.grad + kOffW1, kMlpH, kMlpIn, kMlpH, B, 0, st));
   MLP_HIP(ctx, mlp_sum(m.sqerr, B, m.loss, st));
   return POND_OK;
@@ -283,7 +284,7 @@ pond_status pond_mlp_train(pond_ctx* ctx, int epochs, int batch, float lr, uint6
         ctx->err = "pond_mlp_train: batch gather failed";
         break;
       }
-      st = mlp_fwd_bwd(ctx, batch);
+      st = mlp_fwd_bwd(ctx, batch, false);
       if (st != POND_OK) break;
       m.adam_t++;
       const float c1 = 1.0f - std::pow(b1, (float)m.adam_t), c2 = 1.0f - std::pow(b2, (float)m.adam_t);
@@ -381,7 +382,7 @@ pond_status pond_mlp_grad(pond_ctx* ctx, const float* X, const float* T, int64_t
   if (s != POND_OK) return s;
   MLP_HIP(ctx, hipMemcpy(m.bx, X, (size_t)B * kMlpIn * 4, hipMemcpyHostToDevice));
   MLP_HIP(ctx, hipMemcpy(m.bt, T, (size_t)B * kMlpOut * 4, hipMemcpyHostToDevice));
-  s = mlp_fwd_bwd(ctx, (int)B);
+  s = mlp_fwd_bwd(ctx, (int)B, true);
   if (s != POND_OK) return s;
   MLP_HIP(ctx, hipStreamSynchronize(ctx->stream));
   MLP_HIP(ctx, hipMemcpy(grad, m.grad, (size_t)kMlpParams * 4, hipMemcpyDeviceToHost));

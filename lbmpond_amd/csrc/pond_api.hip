@@ -251,6 +251,9 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
   g.nx = (int)p->nx; g.ny = (int)p->ny; g.nz = (int)p->nz;
   g.n = p->nx * p->ny * p->nz;
   g.plane = round_up(g.n, 64);
+  // the split step addresses nine FP32 population planes through one 32-bit buffer offset
+  if (p->precision == POND_SPLIT && g.plane * 4 * 9 > (int64_t)UINT32_MAX)
+    return bail(POND_ERR_INVALID, "pond_create: POND_SPLIT supports up to 119,304,640 cells (about 492^3)");
   StepArgs& a = ctx->args;
   a.g = g;
   a.dtau = p->dtau;

@@ -17,6 +17,7 @@
 // loads of neighbouring lanes are neighbouring addresses of each population plane.
 // D3Q125 mapping: one 128-lane block per cell, lane = target population (as the reference's
 // 125-thread block), the fixed 3x3x3 neighbourhood staged in LDS.
+#include <type_traits>
 #include "pond_device.h"
 
 namespace pond {
@@ -71,14 +72,69 @@ __device__ __forceinline__ uint32_t node_off(const Grid& g, int x, int y, int z)
   return (uint32_t)cell_index(g, x, y, z) * 8u;
 }
 
+// The same node data as raw buffer resources: population planes 9j .. 9j+8 in resource j, the
+// plane selected by the scalar offset, the per-lane address a single 32-bit offset (a global
+// load needs a 64-bit per-lane address: two VGPRs and a 64-bit add per load).  Resource sizes
+// are exact, so the hardware range check returns 0 rather than reading past an array.  Word 3
+// (0x00020000) is the raw-buffer configuration of the gfx9 family.  Usable while nine FP64
+// planes fit in 4 GiB (plane <= 59,652,323 cells).
+constexpr int kPlanesPerRsrc64 = 9;
+constexpr int kRawBufferWord3 = 0x00020000;
+
+struct NodeBuf {
+  __amdgpu_buffer_rsrc_t f[3], m, P, ok;
+  uint32_t pbytes;                      // bytes per FP64 plane
+  const double* __restrict__ fptr;      // for the out-of-line checked transfer
+  Grid g;
+};
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc64(const void* base, uint64_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)(uint32_t)bytes, kRawBufferWord3);
+}
+
+__device__ __forceinline__ NodeBuf make_nodebuf(const NodeSrc& s) {
+  NodeBuf b;
+  const uint64_t pb = (uint64_t)s.g.plane * 8u;
+  b.pbytes = (uint32_t)pb;
+  for (int j = 0; j < 3; j++) b.f[j] = rsrc64(s.f + (int64_t)j * kPlanesPerRsrc64 * s.g.plane, kPlanesPerRsrc64 * pb);
+  b.m = rsrc64(s.m, 5 * pb);
+  b.P = rsrc64(s.P, pb);
+  b.ok = rsrc64(s.ok, (uint64_t)s.g.plane);
+  b.fptr = s.f;
+  b.g = s.g;
+  return b;
+}
+
+// node accessors: population k, moment plane k, P, finite flag of the node at byte offset boff
+__device__ __forceinline__ double node_f(const NodeSrc& s, int k, uint32_t boff) { return ldp(s.f, k, s.g.plane, boff); }
+__device__ __forceinline__ double node_m(const NodeSrc& s, int k, uint32_t boff) { return ldp(s.m, k, s.g.plane, boff); }
+__device__ __forceinline__ double node_P(const NodeSrc& s, uint32_t boff) { return ldp(s.P, 0, s.g.plane, boff); }
+__device__ __forceinline__ bool node_ok(const NodeSrc& s, uint32_t boff) { return s.ok[boff >> 3] != 0; }
+__device__ __forceinline__ const double* node_fptr(const NodeSrc& s) { return s.f; }
+
+__device__ __forceinline__ double ldb64(__amdgpu_buffer_rsrc_t r, uint32_t boff, uint32_t soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)boff, (int)soff, 0));
+}
+__device__ __forceinline__ double node_f(const NodeBuf& s, int k, uint32_t boff) {
+  return ldb64(s.f[k / kPlanesPerRsrc64], boff, (uint32_t)(k % kPlanesPerRsrc64) * s.pbytes);
+}
+__device__ __forceinline__ double node_m(const NodeBuf& s, int k, uint32_t boff) {
+  return ldb64(s.m, boff, (uint32_t)k * s.pbytes);
+}
+__device__ __forceinline__ double node_P(const NodeBuf& s, uint32_t boff) { return ldb64(s.P, boff, 0); }
+__device__ __forceinline__ bool node_ok(const NodeBuf& s, uint32_t boff) {
+  return __builtin_amdgcn_raw_buffer_load_b8(s.ok, (int)(boff >> 3), 0, 0) != 0;
+}
+__device__ __forceinline__ const double* node_fptr(const NodeBuf& s) { return s.fptr; }
+
 // basis values of the node at byte offset boff for target t (structs.cuh:530-538).
 // e_iold*sqrt(|T0|)*sign(T0) for e_iold = -1, 0, +1 is exactly {-P, 0*P, P}.
-__device__ __forceinline__ bool basis27(const NodeSrc& s, uint32_t boff, const Tgt27& t, double sqTb, double gx[3],
+template <class SRC>
+__device__ __forceinline__ bool basis27(const SRC& s, uint32_t boff, const Tgt27& t, double sqTb, double gx[3],
                                         double gy[3], double gz[3]) {
-  const int64_t plane = s.g.plane;
-  const double u0x = ldp(s.m, 1, plane, boff), u0y = ldp(s.m, 2, plane, boff), u0z = ldp(s.m, 3, plane, boff);
-  const double Pn = ldp(s.P, 0, plane, boff);
-  const bool nok = s.ok[boff >> 3] != 0;
+  const double u0x = node_m(s, 1, boff), u0y = node_m(s, 2, boff), u0z = node_m(s, 3, boff);
+  const double Pn = node_P(s, boff);
+  const bool nok = node_ok(s, boff);
   const double Dx = (t.u1x - u0x) * sqTb;
   const double Dy = (t.u1y - u0y) * sqTb;
   const double Dz = (t.u1z - u0z) * sqTb;
@@ -115,10 +171,11 @@ __device__ __noinline__ double transfer27_checked(const double* __restrict__ f, 
 }
 
 // gauge transfer of the node at byte offset boff to target t (structs.cuh:525-548)
-__device__ __forceinline__ double transfer27(const NodeSrc& s, uint32_t boff, const Tgt27& t, double sqTb) {
+template <class SRC>
+__device__ __forceinline__ double transfer27(const SRC& s, uint32_t boff, const Tgt27& t, double sqTb) {
   double gx[3], gy[3], gz[3];
   const bool safe = basis27(s, boff, t, sqTb, gx, gy, gz);
-  if (!safe) return transfer27_checked(s.f, s.g.plane, boff, t.W, gx, gy, gz);
+  if (!safe) return transfer27_checked(node_fptr(s), s.g.plane, boff, t.W, gx, gy, gz);
   double p2[3][3];
 #pragma unroll
   for (int a = 0; a < 3; a++)
@@ -128,7 +185,7 @@ __device__ __forceinline__ double transfer27(const NodeSrc& s, uint32_t boff, co
 #pragma unroll
   for (int iold = 0; iold < 27; iold++) {
     const int jx = kE27[iold][0] + 1, jy = kE27[iold][1] + 1, jz = kE27[iold][2] + 1;
-    fnew += p2[jx][jy] * gz[jz] * ldp(s.f, iold, s.g.plane, boff);
+    fnew += p2[jx][jy] * gz[jz] * node_f(s, iold, boff);
   }
   return fnew;
 }
@@ -163,10 +220,10 @@ __device__ __forceinline__ Tgt5 make_tgt5(int inew, double u1x, double T1) {
   return t;
 }
 
-__device__ __forceinline__ double transfer5(const NodeSrc& s, uint32_t boff, const Tgt5& t) {
-  const int64_t plane = s.g.plane;
-  const double u0x = ldp(s.m, 1, plane, boff);
-  const double s0 = ldp(s.P, 0, plane, boff);  // sqrts(T0/1.0) = sqrt(|T0|)*sign(T0)
+template <class SRC>
+__device__ __forceinline__ double transfer5(const SRC& s, uint32_t boff, const Tgt5& t) {
+  const double u0x = node_m(s, 1, boff);
+  const double s0 = node_P(s, boff);  // sqrts(T0/1.0) = sqrt(|T0|)*sign(T0)
   const double et = t.et;
   const double difu = t.u1x - u0x;
   double fnew = t.fnew0;
@@ -189,7 +246,7 @@ __device__ __forceinline__ double transfer5(const NodeSrc& s, uint32_t boff, con
     dm4 = dm4 * dm4 * dm4 * dm4;
     dm4 *= 0;
     const double gx = K4 * difu * difu * difu * difu + K3 * difu * difu * difu + K2 * difu * difu + K1 * difu + K0 - dm4;
-    const double fadd = t.wk * gx * ldp(s.f, iold, plane, boff);
+    const double fadd = t.wk * gx * node_f(s, iold, boff);
     if (finite_d(fadd)) fnew += fadd;
   }
   return fnew;
@@ -207,7 +264,8 @@ struct Lat<kD3Q27> {
   using Tgt = Tgt27;
   __device__ static double efc(int i, int d) { return (double)kE27[i][d]; }
   __device__ static Tgt tgt(int i, const double* g) { return make_tgt27(i, g[0], g[1], g[2], g[3]); }
-  __device__ static double xfer(const NodeSrc& s, uint32_t boff, const Tgt& t, double sqTb) {
+  template <class SRC>
+  __device__ static double xfer(const SRC& s, uint32_t boff, const Tgt& t, double sqTb) {
     return transfer27(s, boff, t, sqTb);
   }
 };
@@ -218,7 +276,8 @@ struct Lat<kD1Q5> {
   using Tgt = Tgt5;
   __device__ static double efc(int i, int d) { return d == 0 ? kA5[i] : 0.0; }
   __device__ static Tgt tgt(int i, const double* g) { return make_tgt5(i, g[0], g[3]); }
-  __device__ static double xfer(const NodeSrc& s, uint32_t boff, const Tgt& t, double) {
+  template <class SRC>
+  __device__ static double xfer(const SRC& s, uint32_t boff, const Tgt& t, double) {
     return transfer5(s, boff, t);
   }
 };
@@ -250,8 +309,8 @@ __device__ __forceinline__ void lag4_factors(double s, double F[4][3]) {
 // CENTRED=true: the corrected variant (not in the reference): y/z nodes floor(y+0.5)-1..+1 and
 // smoothness samples on the departure point's own row, which is the middle node of every
 // slice, so they reuse that slice's transfers.
-template <int LAT, bool CENTRED>
-__device__ double interp_weno5(const NodeSrc& src, double cx, double cy, double cz, const typename Lat<LAT>::Tgt& t,
+template <int LAT, bool CENTRED, class SRC>
+__device__ double interp_weno5(const SRC& src, double cx, double cy, double cz, const typename Lat<LAT>::Tgt& t,
                                double sqTb) {
   const Grid& g = src.g;
   const int r = floor_int(cx + 0.5);
@@ -341,8 +400,8 @@ __device__ double interp_weno5(const NodeSrc& src, double cx, double cy, double 
 }
 
 // propagate<4> (steps.cu:986-1036): tricubic Lagrange on floor(x)-1 .. floor(x)+2
-template <int LAT>
-__device__ double interp_lagr4(const NodeSrc& src, double cx, double cy, double cz, const typename Lat<LAT>::Tgt& t,
+template <int LAT, class SRC>
+__device__ double interp_lagr4(const SRC& src, double cx, double cy, double cz, const typename Lat<LAT>::Tgt& t,
                                double sqTb) {
   const Grid& g = src.g;
   const int lx = floor_int(cx) - 4 / 2 + 1;
@@ -381,11 +440,16 @@ __device__ double interp_lagr4(const NodeSrc& src, double cx, double cy, double 
 // ---------------------------------------------------------------------------------------------
 // One lane per cell: LBMpull (steps.cu:356-365, 421-512, 557-564, 582)
 
-template <int LAT, int INTERP>
-__global__ __launch_bounds__(64) void k_step_f64(const NodeSrc src, double* __restrict__ fout,
+// BUF: node loads through buffer resources (planes up to 59,652,323 cells), else global loads
+template <int LAT, int INTERP, bool BUF>
+__global__ __launch_bounds__(64) void k_step_f64(const NodeSrc gsrc, double* __restrict__ fout,
                                                  double* __restrict__ mout, int* __restrict__ niter_out,
                                                  const StepArgs a) {
   constexpr int Q = Lat<LAT>::Q;
+  using Src = typename std::conditional<BUF, NodeBuf, NodeSrc>::type;
+  Src src;
+  if constexpr (BUF) src = make_nodebuf(gsrc);
+  else src = gsrc;
   const Grid& g = a.g;
   const int64_t cell = xcd_block(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
   const bool active = cell < g.n;
@@ -396,7 +460,8 @@ __global__ __launch_bounds__(64) void k_step_f64(const NodeSrc src, double* __re
     const int z = (int)(cell / ((int64_t)g.nx * g.ny));
     const int64_t P = g.plane;
     // c0 = load; all-zero cell -> set_base(1, (0,0,0,TLat)) (steps.cu:360-363)
-    double rho0 = src.m[cell], c0[4] = {src.m[P + cell], src.m[2 * P + cell], src.m[3 * P + cell], src.m[4 * P + cell]};
+    double rho0 = gsrc.m[cell],
+           c0[4] = {gsrc.m[P + cell], gsrc.m[2 * P + cell], gsrc.m[3 * P + cell], gsrc.m[4 * P + cell]};
     const bool zero = rho0 == 0 && c0[0] == 0 && c0[1] == 0 && c0[2] == 0 && c0[3] == 0;
     if (zero) { rho0 = 1; c0[3] = a.tlat; }
     while (niter < a.max_iter) {
@@ -420,7 +485,7 @@ __global__ __launch_bounds__(64) void k_step_f64(const NodeSrc src, double* __re
         m2 += fn * vz;
         m3 += fn * (vx * vx + vy * vy + vz * vz);
         const int64_t o = (int64_t)i * P + cell;
-        const double fo = niter > 1 ? fout[o] : (zero ? a.w[i] * 1.0 : src.f[o]);
+        const double fo = niter > 1 ? fout[o] : (zero ? a.w[i] * 1.0 : gsrc.f[o]);
         if (fabs(fn - fo) >= a.tol_abs + a.tol_rel * fabs(fn)) conv = false;
         fout[o] = fn;
       }
@@ -656,15 +721,24 @@ hipError_t launch_step_f64(int lattice, int interp, const double* fin, const dou
   }
   const unsigned blocks = (unsigned)((a.g.n + 63) / 64);
   const NodeSrc src{fin, min, Pin, okin, a.g};
-#define POND_LAUNCH(L, I) hipLaunchKernelGGL((k_step_f64<L, I>), dim3(blocks), dim3(64), 0, st, src, fout, mout, niter, a)
+  // buffer-resource loads while nine FP64 planes fit a 32-bit offset
+  const bool buf = (uint64_t)a.g.plane * 8u * kPlanesPerRsrc64 <= 0xffffffffull;
+#define POND_LAUNCH(L, I, B) \
+  hipLaunchKernelGGL((k_step_f64<L, I, B>), dim3(blocks), dim3(64), 0, st, src, fout, mout, niter, a)
   if (lattice == kD3Q27) {
-    if (interp == 0) POND_LAUNCH(kD3Q27, 0);
-    else if (interp == 1) POND_LAUNCH(kD3Q27, 1);
-    else POND_LAUNCH(kD3Q27, 2);
+    if (buf) {
+      if (interp == 0) POND_LAUNCH(kD3Q27, 0, true);
+      else if (interp == 1) POND_LAUNCH(kD3Q27, 1, true);
+      else POND_LAUNCH(kD3Q27, 2, true);
+    } else {
+      if (interp == 0) POND_LAUNCH(kD3Q27, 0, false);
+      else if (interp == 1) POND_LAUNCH(kD3Q27, 1, false);
+      else POND_LAUNCH(kD3Q27, 2, false);
+    }
   } else {
     // on a line (Ny = Nz = 1) the centred and as-written WENO5 coincide
-    if (interp == 1) POND_LAUNCH(kD1Q5, 1);
-    else POND_LAUNCH(kD1Q5, 0);
+    if (interp == 1) POND_LAUNCH(kD1Q5, 1, false);
+    else POND_LAUNCH(kD1Q5, 0, false);
   }
 #undef POND_LAUNCH
   return hipGetLastError();

@@ -73,12 +73,47 @@ __device__ __forceinline__ void basis3(float D, float sq0, bool kn, float c, flo
   }
 }
 
-// sum-factorised gauge transfer of one node to target t.  The gauge difference u1 - u0 is
-// formed as (hi - u0) + lo: hi - u0 is exact when the two velocities are within a factor of two
-// (Sterbenz), so the difference carries one rounding relative to itself rather than one relative
-// to |u1|, which would put a jitter of ulp(|u|) into the velocity moment.
-__device__ __forceinline__ float transfer27f(const float* __restrict__ f, int64_t plane, float u0x, float u0y,
-                                             float u0z, float sq0, const TgtF& t, float sqTb) {
+// Population planes as buffer resources: resource j covers planes 9j .. 9j+8, a load names its
+// plane through the scalar offset (k mod 9) * plane bytes, and the per-lane address is a single
+// 32-bit offset, where a global load needs a 64-bit per-lane address (two VGPRs and a 64-bit
+// add per load).  The resource size is exact, so the hardware range check returns 0 rather
+// than reading past the array.  Word 3 (0x00020000) is the raw-buffer configuration of the gfx9
+// family.  Usable while 9 planes fit in 4 GiB (plane <= 119,304,647 cells).
+constexpr int kPlanesPerRsrc = 9;
+constexpr uint32_t kRawBufferWord3 = 0x00020000u;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint64_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)(uint32_t)bytes, (int)kRawBufferWord3);
+}
+
+struct PlanesF {
+  __amdgpu_buffer_rsrc_t r[3];
+  uint32_t pbytes;  // bytes per plane
+};
+
+__device__ __forceinline__ PlanesF make_planes(const float* base, int nplanes, int64_t plane) {
+  PlanesF p;
+  p.pbytes = (uint32_t)(plane * 4);
+  for (int j = 0; j < 3; j++) {
+    const int first = j * kPlanesPerRsrc;
+    const int cnt = nplanes - first < kPlanesPerRsrc ? nplanes - first : kPlanesPerRsrc;
+    p.r[j] = make_rsrc(base + (int64_t)(first < nplanes ? first : 0) * plane, cnt > 0 ? (uint64_t)cnt * p.pbytes : 0);
+  }
+  return p;
+}
+
+__device__ __forceinline__ float ldf(const PlanesF& p, int k, uint32_t boff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(p.r[k / kPlanesPerRsrc], (int)boff,
+                                                                        (int)((uint32_t)(k % kPlanesPerRsrc) * p.pbytes), 0));
+}
+
+// sum-factorised gauge transfer of the node at byte offset boff to target t.  The gauge
+// difference u1 - u0 is formed as (hi - u0) + lo: hi - u0 is exact when the two velocities are
+// within a factor of two (Sterbenz), so the difference carries one rounding relative to itself
+// rather than one relative to |u1|, which would put a jitter of ulp(|u|) into the velocity
+// moment.
+__device__ __forceinline__ float transfer27f(const PlanesF& f, uint32_t boff, float u0x, float u0y, float u0z,
+                                             float sq0, const TgtF& t, float sqTb) {
   float Lx[3], Ly[3], Lz[3];
   basis3(((t.u1x - u0x) + t.u1xl) * sqTb, sq0, t.kx, t.cx, t.T1, t.wx, Lx);
   basis3(((t.u1y - u0y) + t.u1yl) * sqTb, sq0, t.ky, t.cy, t.T1, t.wy, Ly);
@@ -91,7 +126,7 @@ __device__ __forceinline__ float transfer27f(const float* __restrict__ f, int64_
     for (int b = 0; b < 3; b++) {
       float sz = 0.0f;
 #pragma unroll
-      for (int c = 0; c < 3; c++) sz = __builtin_fmaf(Lz[c], f[(int64_t)kIdx27.v[a][b][c] * plane], sz);
+      for (int c = 0; c < 3; c++) sz = __builtin_fmaf(Lz[c], ldf(f, kIdx27.v[a][b][c], boff), sz);
       sy = __builtin_fmaf(Ly[b], sz, sy);
     }
     out = __builtin_fmaf(Lx[a], sy, out);
@@ -99,16 +134,19 @@ __device__ __forceinline__ float transfer27f(const float* __restrict__ f, int64_
   return out;
 }
 
+// node side of the load state: populations, (rho, ux, uy, uz) and sqrt(|T|)*sign(T) planes
 struct NodeSrcF {
-  const float* __restrict__ f;
-  const float* __restrict__ m;   // rho, ux, uy, uz
-  const float* __restrict__ sq;  // sqrt(|T|)*sign(T) per cell
+  PlanesF f, m, sq;
   Grid g;
 };
 
+__device__ __forceinline__ NodeSrcF make_src(const float* f, const float* m, const float* sq, const Grid& g) {
+  return NodeSrcF{make_planes(f, 27, g.plane), make_planes(m, 4, g.plane), make_planes(sq, 1, g.plane), g};
+}
+
 __device__ __forceinline__ float node_xfer(const NodeSrcF& s, int64_t c, const TgtF& t, float sqTb) {
-  const int64_t P = s.g.plane;
-  return transfer27f(s.f + c, P, s.m[P + c], s.m[2 * P + c], s.m[3 * P + c], s.sq[c], t, sqTb);
+  const uint32_t boff = (uint32_t)c * 4u;
+  return transfer27f(s.f, boff, ldf(s.m, 1, boff), ldf(s.m, 2, boff), ldf(s.m, 3, boff), ldf(s.sq, 0, boff), t, sqTb);
 }
 
 __device__ __forceinline__ void lag3f(float s, float F[3]) {
@@ -262,7 +300,7 @@ __global__ __launch_bounds__(64) void k_step_split27(const float* __restrict__ f
       learned = true;
     }
   }
-  const NodeSrcF src{fin, min, sqin, g};
+  const NodeSrcF src = make_src(fin, min, sqin, g);
   const float sqTb = (float)a.sqrt_tbase;
   int niter = 0;
   double res_prev = 0;

@@ -66,6 +66,18 @@ def test_invalid_params_are_rejected_without_touching_the_gpu():
         assert L.pond_last_error(None)
 
 
+def test_split_grid_limit_is_checked_before_allocation():
+    """The split step addresses nine FP32 population planes through a 32-bit buffer offset."""
+    L = lbmpond_amd.lib()
+    p = pondmod.PondParams()
+    L.pond_params_default(ctypes.byref(p), 1)
+    p.precision = 1
+    p.nx = p.ny = p.nz = 500
+    h = ctypes.c_void_p()
+    assert L.pond_create(ctypes.byref(p), ctypes.byref(h)) == 1
+    assert b"POND_SPLIT supports up to" in L.pond_last_error(None)
+
+
 def test_arr_reader_rejects_bad_files(tmp_path):
     bad = tmp_path / "bad.arr"
     bad.write_bytes(b"\x00\x00\x00\x00\x02\x00\x00\x00")
