@@ -37,7 +37,11 @@ FP64_PEAK_TFLOPS = 78.6
 
 # algorithmic bytes per lattice update: read one state, write one state, int32 iterations
 BYTES_PER_LUP = {("D3Q27", "fp64"): 2 * (27 * 8 + 5 * 8) + 4, ("D3Q27", "split"): 2 * (27 * 4 + 4 * 4 + 8) + 4,
-                 ("D3Q125", "fp64"): 2 * (125 * 8 + 5 * 8) + 4, ("D1Q5", "fp64"): 2 * (5 * 8 + 5 * 8) + 4}
+                 ("D3Q125", "fp64"): 2 * (125 * 8 + 5 * 8) + 4, ("D3Q125", "split"): 2 * (125 * 4 + 4 * 4 + 8) + 4,
+                 ("D1Q5", "fp64"): 2 * (5 * 8 + 5 * 8) + 4}
+# the step kernel each path launches for the bench configurations
+KERNEL_NAME = {("D3Q27", "fp64"): "k_step_f64<1,2,true>", ("D3Q27", "split"): "k_step_split27<2>",
+               ("D3Q125", "fp64"): "k_step_q125_f64", ("D3Q125", "split"): "k_step_split125"}
 # algorithmic FP64 flops per cell-iteration of the reference's algorithm with de-duplicated
 # transfers: 19 flops per D3Q27 transfer term, 27 targets x 50 transfers x 27 terms (WENO5)
 FLOP_PER_CELL_ITER = {"D3Q27": 27 * 50 * 27 * 19, "D3Q125": 125 * 27 * 125 * 29, "D1Q5": 5 * 5 * 5 * 30}
@@ -239,7 +243,7 @@ def main():
         "paths": summary,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "bytes_per_lup": bpl,
-                     "kernel": "k_step_split27<2>" if prec == "split" else "k_step_f64<1,2>"},
+                     "kernel": KERNEL_NAME[(case.lattice, prec)]},
         "valu_roofline": {"bound": "fp64-valu", "achieved": round(flops / launch_s / 1e12, 4), "peak": FP64_PEAK_TFLOPS,
                           "unit": "TFLOP/s", "frac": round(flops / launch_s / 1e12 / FP64_PEAK_TFLOPS, 5),
                           "flop_per_cell_iter": FLOP_PER_CELL_ITER[case.lattice],

@@ -239,8 +239,8 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
   if (p->interp != POND_WENO5 && p->interp != POND_LAGR4 && p->interp != POND_WENO5C) return bail(POND_ERR_INVALID, "pond_create: unknown interp");
   if (p->precision != POND_FP64 && p->precision != POND_SPLIT)
     return bail(POND_ERR_INVALID, "pond_create: unknown precision");
-  if (p->precision == POND_SPLIT && p->lattice != POND_D3Q27)
-    return bail(POND_ERR_INVALID, "pond_create: POND_SPLIT is built for D3Q27 only");
+  if (p->precision == POND_SPLIT && p->lattice == POND_D1Q5)
+    return bail(POND_ERR_INVALID, "pond_create: POND_SPLIT is built for D3Q27 and D3Q125");
   if (p->precision == POND_SPLIT && !(p->split_tol_rel >= 0 && p->split_tol_abs >= 0))
     return bail(POND_ERR_INVALID, "pond_create: split tolerances must be >= 0");
   if (p->max_iter < 1) return bail(POND_ERR_INVALID, "pond_create: max_iter must be >= 1");
@@ -252,8 +252,8 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
   g.n = p->nx * p->ny * p->nz;
   g.plane = round_up(g.n, 64);
   // the split step addresses nine FP32 population planes through one 32-bit buffer offset
-  if (p->precision == POND_SPLIT && g.plane * 4 * 9 > (int64_t)UINT32_MAX)
-    return bail(POND_ERR_INVALID, "pond_create: POND_SPLIT supports up to 119,304,640 cells (about 492^3)");
+  if (p->precision == POND_SPLIT && p->lattice == POND_D3Q27 && g.plane * 4 * 9 > (int64_t)UINT32_MAX)
+    return bail(POND_ERR_INVALID, "pond_create: POND_SPLIT D3Q27 supports up to 119,304,640 cells (about 492^3)");
   StepArgs& a = ctx->args;
   a.g = g;
   a.dtau = p->dtau;

@@ -75,7 +75,7 @@ def test_split_grid_limit_is_checked_before_allocation():
     p.nx = p.ny = p.nz = 500
     h = ctypes.c_void_p()
     assert L.pond_create(ctypes.byref(p), ctypes.byref(h)) == 1
-    assert b"POND_SPLIT supports up to" in L.pond_last_error(None)
+    assert b"POND_SPLIT D3Q27 supports up to" in L.pond_last_error(None)
 
 
 def test_arr_reader_rejects_bad_files(tmp_path):

@@ -98,3 +98,15 @@ def test_split_state_roundtrip(gpu):
     assert np.array_equal(out["T"], T)  # T is kept in FP64
     w = np.array([8 / 27] + [0] * 26)
     assert abs(f[0] - (w[0] * rho).astype(np.float32)).max() < 1e-6
+
+
+def test_split_d3q125_random(gpu):
+    """D3Q125 split step (block per cell, fixed 3x3x3 stencil) against the FP64 D3Q125 step."""
+    shape = (6, 5, 4)
+    compare(shape, "lagr4", smooth_random_state(shape, seed=65, T0=0.05, u_amp=0.05), 10, 1e-5, lattice="D3Q125")
+
+
+def test_split_d3q125_turbulence(gpu):
+    case = ic.isotropic_turbulence(n=12)
+    errs, ref, got = compare(case.shape, "lagr4", case.fields, 10, 1e-5, dtau=case.dtau, lattice="D3Q125")
+    assert got["niter_hist"].mean() <= ref["niter_hist"].mean()
