@@ -123,6 +123,10 @@ def run_path(case, precision, learned, collect, warmup, steps, train_cfg):
                mlups=cells * steps / wall / 1e6, niter_mean=float(np.mean(iters)),
                niter_max=int(p.diagnostics()["niter_max"]))
     res["fields"] = p.fields()
+    # a wave of 64 consecutive cells runs as many iterations as its slowest cell
+    it = res["fields"]["niter"].astype(np.float64)
+    pad = (-it.size) % 64
+    res["niter_wave_mean"] = float(np.pad(it, (0, pad)).reshape(-1, 64).max(axis=1).mean())
     p.close()
     return res
 
@@ -203,6 +207,7 @@ def main():
     summary = {}
     for name, r in results.items():
         s = {"mlups": round(r["mlups"], 4), "niter_mean": round(r["niter_mean"], 4), "niter_max": r["niter_max"],
+             "niter_wave_mean_last": round(r["niter_wave_mean"], 4),
              "ms_per_step": round(r["wall_s"] * 1e3 / args.steps, 3),
              "kernel_ms_per_step": round(r["kernel_ms"] / args.steps, 3)}
         if ref is not None and name != "fp64":

@@ -2,6 +2,7 @@
 // download, the step driver (calcStep, update.cu:86-142, without the I/O it times), global
 // diagnostics and the .arr field dump.
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -251,6 +252,23 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
   g.nx = (int)p->nx; g.ny = (int)p->ny; g.nz = (int)p->nz;
   g.n = p->nx * p->ny * p->nz;
   g.plane = round_up(g.n, 64);
+  // wave tile of the one-lane-per-cell step kernels: POND_WAVE_TILE="TXxTYxTZ" (product 64),
+  // 64x1x1 by default and on a line
+  g.tx = 64; g.ty = 1; g.tz = 1;
+  if (const char* wt = std::getenv("POND_WAVE_TILE")) {
+    int a = 0, b = 0, c = 0;
+    if (std::sscanf(wt, "%dx%dx%d", &a, &b, &c) != 3 || a < 1 || b < 1 || c < 1 || a * b * c != 64)
+      return bail(POND_ERR_INVALID, "pond_create: POND_WAVE_TILE must be TXxTYxTZ with TX*TY*TZ = 64");
+    if (g.ny > 1 || g.nz > 1) { g.tx = a; g.ty = b; g.tz = c; }
+  }
+  if (g.tx == 64) {
+    g.ntx = (int)((g.n + 63) / 64);
+    g.nty = g.ntz = 1;
+  } else {
+    g.ntx = (g.nx + g.tx - 1) / g.tx;
+    g.nty = (g.ny + g.ty - 1) / g.ty;
+    g.ntz = (g.nz + g.tz - 1) / g.tz;
+  }
   // the split step addresses nine FP32 population planes through one 32-bit buffer offset
   if (p->precision == POND_SPLIT && p->lattice == POND_D3Q27 && g.plane * 4 * 9 > (int64_t)UINT32_MAX)
     return bail(POND_ERR_INVALID, "pond_create: POND_SPLIT D3Q27 supports up to 119,304,640 cells (about 492^3)");

@@ -12,6 +12,11 @@ struct Grid {
   int nx, ny, nz;
   int64_t n;
   int64_t plane;
+  // 64-cell wave tile of the one-lane-per-cell kernels (tx*ty*tz = 64) and the number of tiles
+  // along each axis; the tiles cover the grid, lanes outside it are idle.  tx = 64: linear
+  // order, ntx*nty*ntz = ceil(n/64) waves
+  int tx, ty, tz;
+  int ntx, nty, ntz;
 };
 
 // Parameters of one step, passed by value to the kernels.
@@ -62,6 +67,24 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks) {
   const int64_t q = nblocks / kXcd, r = nblocks % kXcd;
   const int64_t x = b % kXcd, local = b / kXcd;
   return x * q + (x < r ? x : r) + local;
+}
+
+// cell (x, y, z) of lane `lane` in tile `b` (tiles x fastest); false outside the grid.  The
+// 64x1x1 tile is the linear order: wave b holds cells 64b .. 64b+63, across row ends.
+__device__ __forceinline__ bool tile_cell(const Grid& g, int64_t b, int lane, int& x, int& y, int& z) {
+  if (g.tx == 64) {
+    const int64_t c = b * 64 + lane;
+    x = (int)(c % g.nx);
+    y = (int)((c / g.nx) % g.ny);
+    z = (int)(c / ((int64_t)g.nx * g.ny));
+    return c < g.n;
+  }
+  const int lx = lane % g.tx, ly = (lane / g.tx) % g.ty, lz = lane / (g.tx * g.ty);
+  const int64_t bx = b % g.ntx, r = b / g.ntx;
+  x = (int)bx * g.tx + lx;
+  y = (int)(r % g.nty) * g.ty + ly;
+  z = (int)(r / g.nty) * g.tz + lz;
+  return x < g.nx && y < g.ny && z < g.nz;
 }
 
 __device__ __forceinline__ int64_t cell_index(const Grid& g, int x, int y, int z) {

@@ -451,13 +451,11 @@ __global__ __launch_bounds__(64) void k_step_f64(const NodeSrc gsrc, double* __r
   if constexpr (BUF) src = make_nodebuf(gsrc);
   else src = gsrc;
   const Grid& g = a.g;
-  const int64_t cell = xcd_block(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
-  const bool active = cell < g.n;
+  int x, y, z;
+  const bool active = tile_cell(g, xcd_block(blockIdx.x, gridDim.x), threadIdx.x, x, y, z);
+  const int64_t cell = cell_index(g, x, y, z);
   int niter = 0;
   if (active) {
-    const int x = (int)(cell % g.nx);
-    const int y = (int)((cell / g.nx) % g.ny);
-    const int z = (int)(cell / ((int64_t)g.nx * g.ny));
     const int64_t P = g.plane;
     // c0 = load; all-zero cell -> set_base(1, (0,0,0,TLat)) (steps.cu:360-363)
     double rho0 = gsrc.m[cell],
@@ -719,7 +717,7 @@ hipError_t launch_step_f64(int lattice, int interp, const double* fin, const dou
                        a);
     return hipGetLastError();
   }
-  const unsigned blocks = (unsigned)((a.g.n + 63) / 64);
+  const unsigned blocks = (unsigned)((int64_t)a.g.ntx * a.g.nty * a.g.ntz);
   const NodeSrc src{fin, min, Pin, okin, a.g};
   // buffer-resource loads while nine FP64 planes fit a 32-bit offset
   const bool buf = (uint64_t)a.g.plane * 8u * kPlanesPerRsrc64 <= 0xffffffffull;

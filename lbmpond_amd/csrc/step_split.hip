@@ -278,12 +278,10 @@ __global__ __launch_bounds__(64) void k_step_split27(const float* __restrict__ f
                                                      double* __restrict__ Tout, int* __restrict__ niter_out,
                                                      const float* __restrict__ guess, const SplitArgs a) {
   const Grid& g = a.g;
-  const int64_t cell = xcd_block(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
-  if (cell >= g.n) return;
+  int x, y, z;
+  if (!tile_cell(g, xcd_block(blockIdx.x, gridDim.x), threadIdx.x, x, y, z)) return;
+  const int64_t cell = cell_index(g, x, y, z);
   const int64_t P = g.plane;
-  const int x = (int)(cell % g.nx);
-  const int y = (int)((cell / g.nx) % g.ny);
-  const int z = (int)(cell / ((int64_t)g.nx * g.ny));
   double rho0 = min[cell];
   double gp[4] = {min[P + cell], min[2 * P + cell], min[3 * P + cell], Tin[cell]};
   if (rho0 == 0 && gp[0] == 0 && gp[1] == 0 && gp[2] == 0 && gp[3] == 0) { rho0 = 1; gp[3] = a.tlat; }
@@ -606,7 +604,7 @@ hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const
     return hipGetLastError();
   }
   if (lattice != kD3Q27) return hipErrorInvalidValue;
-  const unsigned blocks = (unsigned)((a.g.n + 63) / 64);
+  const unsigned blocks = (unsigned)((int64_t)a.g.ntx * a.g.nty * a.g.ntz);
 #define POND_LAUNCH_S(I)                                                                                        \
   hipLaunchKernelGGL((k_step_split27<I>), dim3(blocks), dim3(64), 0, st, in.f, in.m, in.T, sq, out.f, out.m, out.T, \
                      niter, guess, a)

@@ -49,8 +49,15 @@ def test_split_lagr4_random(gpu):
 
 
 def test_split_taylor_green_small(gpu):
+    case = ic.taylor_green(n=32, steps=20)
+    compare(case.shape, "weno5c", case.fields, 20, 1e-5, dtau=case.dtau)
+
+
+def test_split_taylor_green_error_tracks_fp64_sensitivity(gpu):
+    """Re 1600 Taylor-Green (dtau near 2, little dissipation) over 50 steps: bounded like the
+    vortex case by the FP64 path's own response to a 1e-7 density perturbation."""
     case = ic.taylor_green(n=32, steps=50)
-    compare(case.shape, "weno5c", case.fields, 50, 1e-5, dtau=case.dtau)
+    sensitivity_bound(case, 50)
 
 
 def test_split_vortex_small(gpu):
@@ -61,18 +68,21 @@ def test_split_vortex_small(gpu):
 def test_split_vortex_error_tracks_fp64_sensitivity(gpu):
     """Over 30 steps the vortex case amplifies any perturbation about a hundredfold: its seeded
     1e-3 density noise puts the WENO smoothness indicators near their 1e-6 regularisation, and
-    the nonlinear weights respond to that.  The split path's FP32 rounding (~1e-7 per step) is
-    amplified the same way.  It enters at every step, and what enters at step k is amplified
-    over the remaining steps at most as much as a perturbation of the initial state, so the
-    split error after n steps is bounded by n times the FP64 path's own response to a 1e-7
-    perturbation of the initial density (with a factor 2 for the noise floor, up to 2e-7)."""
-    case = ic.vortex_3d(n=32)
-    steps = 30
-    _, ref, got = compare(case.shape, "weno5c", case.fields, steps, np.inf)
+    the nonlinear weights respond to that."""
+    sensitivity_bound(ic.vortex_3d(n=32), 30)
+
+
+def sensitivity_bound(case, steps):
+    """The split path's FP32 rounding (~1e-7 per step) enters at every step, and what enters at
+    step k is amplified over the remaining steps at most as much as a perturbation of the
+    initial state, so the split error after n steps is bounded by n times the FP64 path's own
+    response to a 1e-7 perturbation of the initial density (with a factor 2 for the noise
+    floor, up to 2e-7)."""
+    _, ref, got = compare(case.shape, "weno5c", case.fields, steps, np.inf, dtau=case.dtau)
     rho, ux, uy, uz, T = case.fields
     rng = np.random.default_rng(17)
     pert = run("D3Q27", case.shape, "weno5c", "fp64", (rho * (1 + 1e-7 * rng.uniform(-1, 1, rho.shape)), ux, uy, uz, T),
-               steps)
+               steps, dtau=case.dtau)
     e_split = state_errors(got, ref)
     e_pert = state_errors(pert, ref)
     print("  fp64 response to a 1e-7 density perturbation:", {k: f"{v:.2e}" for k, v in e_pert.items()})
