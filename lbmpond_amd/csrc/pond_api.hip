@@ -281,6 +281,9 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
   a.sqrt_tbase = std::sqrt(ctx->L.TLat);
   a.tlat = ctx->L.TLat;
   a.inv_dim = 1. / ctx->L.Dim;
+  // iterations of the first pass of the two-pass step (POND_PASS1_ITERS overrides)
+  a.pass1_iters = 3;
+  if (const char* k = std::getenv("POND_PASS1_ITERS")) a.pass1_iters = std::atoi(k) > 0 ? std::atoi(k) : 3;
   for (int i = 0; i < ctx->Qn; i++) a.w[i] = ctx->L.w[i];
 
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
@@ -319,6 +322,7 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
         return bail(POND_ERR_HIP, "pond_create: hipEventCreate failed");
     }
     SplitArgs& sa = ctx->sargs;
+    sa.pass1_iters = a.pass1_iters;
     sa.g = g;
     sa.dtau = p->dtau;
     sa.tol_abs = p->split_tol_abs;
@@ -331,6 +335,11 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
   }
   if (hipMalloc(&ctx->niter, (size_t)g.plane * sizeof(int)) != hipSuccess)
     return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of niter failed");
+  ctx->pend_cap = g.plane;
+  const size_t pend_bytes = 256 + (size_t)g.plane * (4 + 4 * 8 + 8 + 4);
+  if (hipMalloc(&ctx->pend, pend_bytes) != hipSuccess)
+    return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of the pending-cell list failed");
+  ctx->bytes += (int64_t)pend_bytes;
   ctx->red_len = kRedBlocks + 1;
   if (hipMalloc(&ctx->red, (size_t)ctx->red_len * sizeof(RedOut)) != hipSuccess)
     return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of the reduction buffer failed");
@@ -360,6 +369,7 @@ void pond_destroy(pond_ctx* ctx) {
   if (ctx->nodeP) (void)hipFree(ctx->nodeP);
   if (ctx->nodeok) (void)hipFree(ctx->nodeok);
   if (ctx->guess) (void)hipFree(ctx->guess);
+  if (ctx->pend) (void)hipFree(ctx->pend);
   if (ctx->red) (void)hipFree(ctx->red);
   for (auto& e : ctx->ev_pool) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
   for (auto& e : ctx->ev_pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
@@ -462,6 +472,20 @@ pond_status pond_set_populations(pond_ctx* ctx, const double* f, const double* r
   return POND_OK;
 }
 
+// the pending-cell list as the step kernels see it (pond_ctx::pend layout)
+static SplitPending pending_view(pond_ctx* ctx) {
+  const int64_t cap = ctx->pend_cap;
+  char* b = ctx->pend;
+  SplitPending v;
+  v.count = (int*)b;
+  v.g = (double*)(b + 256);
+  v.res = v.g + 4 * cap;
+  v.cell = (int*)(v.res + cap);
+  v.meta = v.cell + cap;
+  v.cap = cap;
+  return v;
+}
+
 pond_status pond_step(pond_ctx* ctx, int nsteps) {
   if (!ctx) return POND_ERR_INVALID;
   if (!ctx->has_state) return fail(ctx, POND_ERR_STATE, "pond_step: no state set");
@@ -502,10 +526,10 @@ pond_status pond_step(pond_ctx* ctx, int nsteps) {
       const SplitPtrs pin{ctx->sp[in].f, ctx->sp[in].m, ctx->sp[in].T};
       const SplitPtrs pout{ctx->sp[outb].f, ctx->sp[outb].m, ctx->sp[outb].T};
       he = launch_step_split(ctx->p.lattice, ctx->p.interp, pin, pout, ctx->sq, ctx->niter,
-                             learned ? ctx->guess : nullptr, ctx->sargs, ctx->stream);
+                             learned ? ctx->guess : nullptr, pending_view(ctx), ctx->sargs, ctx->stream);
     } else {
       he = launch_step_f64(ctx->p.lattice, ctx->p.interp, ctx->f64[in].f, ctx->f64[in].m, ctx->nodeP, ctx->nodeok,
-                           ctx->f64[outb].f, ctx->f64[outb].m, ctx->niter, ctx->args, ctx->stream);
+                           ctx->f64[outb].f, ctx->f64[outb].m, ctx->niter, pending_view(ctx), ctx->args, ctx->stream);
     }
     if (he != hipSuccess) return fail(ctx, POND_ERR_HIP, std::string("step kernel launch: ") + hipGetErrorString(he));
     POND_HIP(ctx, hipEventRecord(e.b, ctx->stream));

@@ -59,8 +59,8 @@ struct MlpCtx {
 hipError_t launch_node_prep(int lattice, const double* f, const double* m, double* P, unsigned char* ok, int64_t n,
                             int64_t plane, hipStream_t st);
 hipError_t launch_step_f64(int lattice, int interp, const double* fin, const double* min, const double* Pin,
-                           const unsigned char* okin, double* fout, double* mout, int* niter, const StepArgs& a,
-                           hipStream_t st);
+                           const unsigned char* okin, double* fout, double* mout, int* niter, const Pending& pend,
+                           const StepArgs& a, hipStream_t st);
 
 }  // namespace pond
 
@@ -81,6 +81,10 @@ struct pond_ctx {
   unsigned char* nodeok = nullptr;  // plain path: populations finite and below 2^200
   float* sq = nullptr;      // split path: sqrt(|T|)*sign(T) per cell of the load buffer
   float* guess = nullptr;   // split path: learned starting gauge increments (4 planes)
+  // cells left unconverged by the first pass of a two-pass step: count, then per entry the
+  // cell index, 4 gauge doubles, one double and one int (48 B per cell of the grid)
+  char* pend = nullptr;
+  int64_t pend_cap = 0;
   int* niter = nullptr;     // per-cell gauge iterations of the last step
   double* red = nullptr;    // reduction partials
   int64_t red_len = 0;

@@ -28,8 +28,37 @@ struct StepArgs {
   double sqrt_tbase;  // sqrt(TLat), evaluated once on the host (structs.cuh:530)
   double tlat;
   double inv_dim;     // 1./Dim (structs.cuh:330)
+  int pass1_iters;    // iterations of the first pass of the one-lane-per-cell step
   double w[kQmax];    // lattice weights for the collide and set_base
 };
+
+// Cells left unconverged by the first pass of a two-pass step and their iteration state,
+// structure of arrays with one entry per cell of the grid.
+struct Pending {
+  int* count;     // number of entries (device)
+  int* cell;      // cell index
+  double* g;      // gauge, 4 planes of cap entries
+  double* res;    // one double per entry (split: residual of the last iteration; FP64: rho)
+  int* meta;      // iteration count (| learned << 16 in the split step)
+  int64_t cap;    // capacity
+};
+
+// Append the pending lanes of a wave to the list: one atomic per wave, entries in lane order.
+// Returns the lane's entry index (valid where pending).
+__device__ __forceinline__ int64_t pending_slot(const Pending& p, bool pending, bool& any) {
+  const unsigned long long mask = __ballot(pending);
+  any = mask != 0;
+  if (!any) return 0;
+  const int lane = threadIdx.x & 63;
+  const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+  const int leader = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(p.count, __popcll(mask));
+  base = __shfl(base, leader, 64);
+  return (int64_t)base + rank;
+}
+
+__global__ void k_zero_count(int* c);
 
 __device__ __forceinline__ int isign_d(double v) { return (0.0 < v) - (v < 0.0); }
 

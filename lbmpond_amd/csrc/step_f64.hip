@@ -440,12 +440,94 @@ __device__ double interp_lagr4(const SRC& src, double cx, double cy, double cz, 
 // ---------------------------------------------------------------------------------------------
 // One lane per cell: LBMpull (steps.cu:356-365, 421-512, 557-564, 582)
 
-// BUF: node loads through buffer resources (planes up to 59,652,323 cells), else global loads
-template <int LAT, int INTERP, bool BUF>
-__global__ __launch_bounds__(64) void k_step_f64(const NodeSrc gsrc, double* __restrict__ fout,
-                                                 double* __restrict__ mout, int* __restrict__ niter_out,
-                                                 const StepArgs a) {
+// Iteration state of one cell: gauge c0, density rho0, count niter.  The iterate's
+// populations live in fout.
+struct F64Iter {
+  double c0[4];
+  double rho0;
+  int niter;
+};
+
+// Fixed-point iterations (LBMpull, steps.cu:421-512) from state st until convergence (true) or
+// until the count reaches stop_at (false).
+template <int LAT, int INTERP, class SRC>
+__device__ bool f64_iterate(const SRC& src, const NodeSrc& gsrc, double* __restrict__ fout, int64_t cell, int x, int y,
+                            int z, bool zero, F64Iter& st, int stop_at, const StepArgs& a) {
   constexpr int Q = Lat<LAT>::Q;
+  const int64_t P = a.g.plane;
+  double* c0 = st.c0;
+  while (st.niter < stop_at) {
+    st.niter++;
+    const int niter = st.niter;
+    const double uT[4] = {c0[0], c0[1], c0[2], c0[3]};
+    const double s = sqrt(uT[3] / a.tlat);
+    double rho = 0, m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+    bool conv = true;
+    for (int i = 0; i < Q; i++) {
+      const double vx = s * Lat<LAT>::efc(i, 0) + uT[0];
+      const double vy = s * Lat<LAT>::efc(i, 1) + uT[1];
+      const double vz = s * Lat<LAT>::efc(i, 2) + uT[2];
+      const typename Lat<LAT>::Tgt t = Lat<LAT>::tgt(i, uT);
+      const double cx = (double)x - vx, cy = (double)y - vy, cz = (double)z - vz;
+      const double fn = INTERP == 1   ? interp_lagr4<LAT>(src, cx, cy, cz, t, a.sqrt_tbase)
+                        : INTERP == 2 ? interp_weno5<LAT, true>(src, cx, cy, cz, t, a.sqrt_tbase)
+                                      : interp_weno5<LAT, false>(src, cx, cy, cz, t, a.sqrt_tbase);
+      rho += fn;
+      m0 += fn * vx;
+      m1 += fn * vy;
+      m2 += fn * vz;
+      m3 += fn * (vx * vx + vy * vy + vz * vz);
+      const int64_t o = (int64_t)i * P + cell;
+      const double fo = niter > 1 ? fout[o] : (zero ? a.w[i] * 1.0 : gsrc.f[o]);
+      if (fabs(fn - fo) >= a.tol_abs + a.tol_rel * fabs(fn)) conv = false;
+      fout[o] = fn;
+    }
+    // Cell::calcMoments (structs.cuh:329-330)
+    if (rho != 0) { m0 /= rho; m1 /= rho; m2 /= rho; m3 /= rho; }
+    m3 = (m3 - m0 * m0 - m1 * m1 - m2 * m2) * a.inv_dim;
+    if (m3 < 0) m3 = c0[3];  // steps.cu:475-478
+    const double mn[4] = {m0, m1, m2, m3};
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (fabs(mn[k] - c0[k]) >= a.tol_abs + a.tol_rel * fabs(mn[k])) conv = false;
+    st.rho0 = rho;
+    c0[0] = m0; c0[1] = m1; c0[2] = m2; c0[3] = m3;
+    if (conv) return true;
+  }
+  return false;
+}
+
+// BGK towards rho*w_i in the converged gauge (steps.cu:557-564) and the cell's outputs
+template <int Q>
+__device__ __forceinline__ void f64_finish(double* __restrict__ fout, double* __restrict__ mout,
+                                           int* __restrict__ niter_out, int64_t cell, const F64Iter& st,
+                                           const StepArgs& a) {
+  const int64_t P = a.g.plane;
+  for (int i = 0; i < Q; i++) {
+    const int64_t o = (int64_t)i * P + cell;
+    double f = fout[o];
+    const double feq = st.rho0 * a.w[i];
+    f += a.dtau * (feq - f);
+    fout[o] = f;
+  }
+  mout[cell] = st.rho0;
+  mout[P + cell] = st.c0[0];
+  mout[2 * P + cell] = st.c0[1];
+  mout[3 * P + cell] = st.c0[2];
+  mout[4 * P + cell] = st.c0[3];
+  niter_out[cell] = st.niter;
+}
+
+// First pass: every cell iterates up to a.pass1_iters times; cells still unconverged are
+// appended to the pending list (one atomic per wave) and finished by k_step_f64_rest, whose
+// waves hold only such cells.  A wave runs as long as its slowest cell, and a thin layer of
+// cells on shocks and contact discontinuities needs several times the mean count.  Each cell's
+// arithmetic and operation order are those of a single pass.
+// BUF: node loads through buffer resources (planes up to 59,652,323 cells), else global loads.
+template <int LAT, int INTERP, bool BUF>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_step_f64(const NodeSrc gsrc, double* __restrict__ fout,
+                                                 double* __restrict__ mout, int* __restrict__ niter_out, Pending pend,
+                                                 const StepArgs a) {
   using Src = typename std::conditional<BUF, NodeBuf, NodeSrc>::type;
   Src src;
   if constexpr (BUF) src = make_nodebuf(gsrc);
@@ -453,66 +535,57 @@ __global__ __launch_bounds__(64) void k_step_f64(const NodeSrc gsrc, double* __r
   const Grid& g = a.g;
   int x, y, z;
   const bool active = tile_cell(g, xcd_block(blockIdx.x, gridDim.x), threadIdx.x, x, y, z);
-  const int64_t cell = cell_index(g, x, y, z);
-  int niter = 0;
+  const int64_t cell = active ? cell_index(g, x, y, z) : 0;
+  bool pending = false;
+  F64Iter st;
   if (active) {
     const int64_t P = g.plane;
     // c0 = load; all-zero cell -> set_base(1, (0,0,0,TLat)) (steps.cu:360-363)
-    double rho0 = gsrc.m[cell],
-           c0[4] = {gsrc.m[P + cell], gsrc.m[2 * P + cell], gsrc.m[3 * P + cell], gsrc.m[4 * P + cell]};
-    const bool zero = rho0 == 0 && c0[0] == 0 && c0[1] == 0 && c0[2] == 0 && c0[3] == 0;
-    if (zero) { rho0 = 1; c0[3] = a.tlat; }
-    while (niter < a.max_iter) {
-      niter++;
-      const double uT[4] = {c0[0], c0[1], c0[2], c0[3]};
-      const double s = sqrt(uT[3] / a.tlat);
-      double rho = 0, m0 = 0, m1 = 0, m2 = 0, m3 = 0;
-      bool conv = true;
-      for (int i = 0; i < Q; i++) {
-        const double vx = s * Lat<LAT>::efc(i, 0) + uT[0];
-        const double vy = s * Lat<LAT>::efc(i, 1) + uT[1];
-        const double vz = s * Lat<LAT>::efc(i, 2) + uT[2];
-        const typename Lat<LAT>::Tgt t = Lat<LAT>::tgt(i, uT);
-        const double cx = (double)x - vx, cy = (double)y - vy, cz = (double)z - vz;
-        const double fn = INTERP == 1   ? interp_lagr4<LAT>(src, cx, cy, cz, t, a.sqrt_tbase)
-                          : INTERP == 2 ? interp_weno5<LAT, true>(src, cx, cy, cz, t, a.sqrt_tbase)
-                                        : interp_weno5<LAT, false>(src, cx, cy, cz, t, a.sqrt_tbase);
-        rho += fn;
-        m0 += fn * vx;
-        m1 += fn * vy;
-        m2 += fn * vz;
-        m3 += fn * (vx * vx + vy * vy + vz * vz);
-        const int64_t o = (int64_t)i * P + cell;
-        const double fo = niter > 1 ? fout[o] : (zero ? a.w[i] * 1.0 : gsrc.f[o]);
-        if (fabs(fn - fo) >= a.tol_abs + a.tol_rel * fabs(fn)) conv = false;
-        fout[o] = fn;
-      }
-      // Cell::calcMoments (structs.cuh:329-330)
-      if (rho != 0) { m0 /= rho; m1 /= rho; m2 /= rho; m3 /= rho; }
-      m3 = (m3 - m0 * m0 - m1 * m1 - m2 * m2) * a.inv_dim;
-      if (m3 < 0) m3 = c0[3];  // steps.cu:475-478
-      const double mn[4] = {m0, m1, m2, m3};
-#pragma unroll
-      for (int k = 0; k < 4; k++)
-        if (fabs(mn[k] - c0[k]) >= a.tol_abs + a.tol_rel * fabs(mn[k])) conv = false;
-      rho0 = rho;
-      c0[0] = m0; c0[1] = m1; c0[2] = m2; c0[3] = m3;
-      if (conv) break;
-    }
-    // BGK towards rho*w_i in the converged gauge (steps.cu:557-564)
-    for (int i = 0; i < Q; i++) {
-      const int64_t o = (int64_t)i * P + cell;
-      double f = fout[o];
-      const double feq = rho0 * a.w[i];
-      f += a.dtau * (feq - f);
-      fout[o] = f;
-    }
-    mout[cell] = rho0;
-    mout[P + cell] = c0[0];
-    mout[2 * P + cell] = c0[1];
-    mout[3 * P + cell] = c0[2];
-    mout[4 * P + cell] = c0[3];
-    niter_out[cell] = niter;
+    st.rho0 = gsrc.m[cell];
+    st.c0[0] = gsrc.m[P + cell]; st.c0[1] = gsrc.m[2 * P + cell]; st.c0[2] = gsrc.m[3 * P + cell];
+    st.c0[3] = gsrc.m[4 * P + cell];
+    const bool zero = st.rho0 == 0 && st.c0[0] == 0 && st.c0[1] == 0 && st.c0[2] == 0 && st.c0[3] == 0;
+    if (zero) { st.rho0 = 1; st.c0[3] = a.tlat; }
+    st.niter = 0;
+    const int stop = a.pass1_iters < a.max_iter ? a.pass1_iters : a.max_iter;
+    const bool conv = f64_iterate<LAT, INTERP>(src, gsrc, fout, cell, x, y, z, zero, st, stop, a);
+    pending = !conv && st.niter < a.max_iter;
+    if (!pending) f64_finish<Lat<LAT>::Q>(fout, mout, niter_out, cell, st, a);
+  }
+  bool any;
+  const int64_t e = pending_slot(pend, pending, any);
+  if (pending) {
+    pend.cell[e] = (int)cell;
+    for (int k = 0; k < 4; k++) pend.g[k * pend.cap + e] = st.c0[k];
+    pend.res[e] = st.rho0;
+    pend.meta[e] = st.niter;
+  }
+}
+
+// Second pass: a fixed grid of waves walks the pending list, every cell resumes its iteration
+template <int LAT, int INTERP, bool BUF>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_step_f64_rest(const NodeSrc gsrc, double* __restrict__ fout,
+                                                      double* __restrict__ mout, int* __restrict__ niter_out,
+                                                      Pending pend, const StepArgs a) {
+  using Src = typename std::conditional<BUF, NodeBuf, NodeSrc>::type;
+  Src src;
+  if constexpr (BUF) src = make_nodebuf(gsrc);
+  else src = gsrc;
+  const Grid& g = a.g;
+  const int count = *pend.count;
+  for (int64_t e = (int64_t)blockIdx.x * 64 + threadIdx.x; e - threadIdx.x < count; e += (int64_t)gridDim.x * 64) {
+    if (e >= count) continue;
+    const int64_t cell = pend.cell[e];
+    const int x = (int)(cell % g.nx);
+    const int y = (int)((cell / g.nx) % g.ny);
+    const int z = (int)(cell / ((int64_t)g.nx * g.ny));
+    F64Iter st;
+    for (int k = 0; k < 4; k++) st.c0[k] = pend.g[k * pend.cap + e];
+    st.rho0 = pend.res[e];
+    st.niter = pend.meta[e];
+    // the zero-cell start only matters for the first iteration, which the first pass ran
+    f64_iterate<LAT, INTERP>(src, gsrc, fout, cell, x, y, z, false, st, a.max_iter, a);
+    f64_finish<Lat<LAT>::Q>(fout, mout, niter_out, cell, st, a);
   }
 }
 
@@ -709,8 +782,8 @@ hipError_t launch_node_prep(int lattice, const double* f, const double* m, doubl
 }
 
 hipError_t launch_step_f64(int lattice, int interp, const double* fin, const double* min, const double* Pin,
-                           const unsigned char* okin, double* fout, double* mout, int* niter, const StepArgs& a,
-                           hipStream_t st) {
+                           const unsigned char* okin, double* fout, double* mout, int* niter, const Pending& pend,
+                           const StepArgs& a, hipStream_t st) {
   if (lattice == kD3Q125) {
     const int64_t blocks = a.g.n;
     hipLaunchKernelGGL(k_step_q125_f64, dim3((unsigned)blocks), dim3(128), 0, st, fin, min, okin, fout, mout, niter,
@@ -721,8 +794,15 @@ hipError_t launch_step_f64(int lattice, int interp, const double* fin, const dou
   const NodeSrc src{fin, min, Pin, okin, a.g};
   // buffer-resource loads while nine FP64 planes fit a 32-bit offset
   const bool buf = (uint64_t)a.g.plane * 8u * kPlanesPerRsrc64 <= 0xffffffffull;
-#define POND_LAUNCH(L, I, B) \
-  hipLaunchKernelGGL((k_step_f64<L, I, B>), dim3(blocks), dim3(64), 0, st, src, fout, mout, niter, a)
+  hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, st, pend.count);
+  const int64_t w2 = (a.g.n + 63) / 64;
+  const unsigned blocks2 = (unsigned)(w2 < 16384 ? w2 : 16384);
+#define POND_LAUNCH(L, I, B)                                                                                     \
+  do {                                                                                                           \
+    hipLaunchKernelGGL((k_step_f64<L, I, B>), dim3(blocks), dim3(64), 0, st, src, fout, mout, niter, pend, a);   \
+    hipLaunchKernelGGL((k_step_f64_rest<L, I, B>), dim3(blocks2), dim3(64), 0, st, src, fout, mout, niter, pend, \
+                       a);                                                                                       \
+  } while (0)
   if (lattice == kD3Q27) {
     if (buf) {
       if (interp == 0) POND_LAUNCH(kD3Q27, 0, true);

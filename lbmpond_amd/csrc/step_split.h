@@ -13,6 +13,7 @@ struct SplitArgs {
   double sqrt_tbase;
   double tlat;
   double inv_dim;
+  int pass1_iters;  // iterations of the first pass of the D3Q27 step (the rest go to the second)
   float w[kQmax];
 };
 
@@ -22,8 +23,11 @@ struct SplitPtrs {
   double* T;  // temperature plane
 };
 
+using SplitPending = Pending;
+
 hipError_t launch_sqrt_t(const double* T, float* sq, int64_t n, hipStream_t st);
 hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const SplitPtrs& out, const float* sq,
-                             int* niter, const float* guess, const SplitArgs& a, hipStream_t st);
+                             int* niter, const float* guess, const SplitPending& pend, const SplitArgs& a,
+                             hipStream_t st);
 
 }  // namespace pond

@@ -271,40 +271,33 @@ __global__ void k_sqrt_t(const double* __restrict__ T, float* __restrict__ sq, i
 // contact discontinuities), whose converged state can depend on the starting point.
 constexpr int kLearnedBudget = 12;
 
+// Iteration state of one cell, carried from the first pass to the second.
+struct SplitIter {
+  double gc[4];      // current gauge (u, T)
+  double res_prev;   // residual of the previous iteration
+  double rho;        // density of the last iterate
+  int niter;
+  bool learned;      // started from the predicted gauge and not yet fallen back
+};
+
+// previous gauge of the cell (the plain start); an all-zero cell is reset as in steps.cu:360-363
+__device__ __forceinline__ void split_prev_gauge(const float* __restrict__ min, const double* __restrict__ Tin,
+                                                 int64_t P, int64_t cell, double tlat, double gp[4], double& rho0) {
+  rho0 = min[cell];
+  gp[0] = min[P + cell]; gp[1] = min[2 * P + cell]; gp[2] = min[3 * P + cell]; gp[3] = Tin[cell];
+  if (rho0 == 0 && gp[0] == 0 && gp[1] == 0 && gp[2] == 0 && gp[3] == 0) { rho0 = 1; gp[3] = tlat; }
+}
+
+// Fixed-point iterations of one cell from state st until it converges (true) or its count
+// reaches stop_at (false).  The iterate's populations are left in fout.
 template <int INTERP>
-__global__ __launch_bounds__(64) void k_step_split27(const float* __restrict__ fin, const float* __restrict__ min,
-                                                     const double* __restrict__ Tin, const float* __restrict__ sqin,
-                                                     float* __restrict__ fout, float* __restrict__ mout,
-                                                     double* __restrict__ Tout, int* __restrict__ niter_out,
-                                                     const float* __restrict__ guess, const SplitArgs a) {
-  const Grid& g = a.g;
-  int x, y, z;
-  if (!tile_cell(g, xcd_block(blockIdx.x, gridDim.x), threadIdx.x, x, y, z)) return;
-  const int64_t cell = cell_index(g, x, y, z);
-  const int64_t P = g.plane;
-  double rho0 = min[cell];
-  double gp[4] = {min[P + cell], min[2 * P + cell], min[3 * P + cell], Tin[cell]};
-  if (rho0 == 0 && gp[0] == 0 && gp[1] == 0 && gp[2] == 0 && gp[3] == 0) { rho0 = 1; gp[3] = a.tlat; }
-  double gc[4] = {gp[0], gp[1], gp[2], gp[3]};
-  bool learned = false;
-  if (guess) {
-    // decode the predicted increment: (du/sqrt(T), dT/T) relative to the previous gauge
-    const double st = sqrt(gp[3]);
-    const double q[4] = {gp[0] + st * guess[cell], gp[1] + st * guess[P + cell], gp[2] + st * guess[2 * P + cell],
-                         gp[3] * (1.0 + (double)guess[3 * P + cell])};
-    const bool ok = isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0;
-    if (ok) {
-      for (int k = 0; k < 4; k++) gc[k] = q[k];
-      learned = true;
-    }
-  }
-  const NodeSrcF src = make_src(fin, min, sqin, g);
+__device__ bool split27_iterate(const NodeSrcF& src, float* __restrict__ fout, int64_t cell, int x, int y, int z,
+                                const double gp[4], SplitIter& st, int stop_at, const SplitArgs& a) {
+  const int64_t P = a.g.plane;
   const float sqTb = (float)a.sqrt_tbase;
-  int niter = 0;
-  double res_prev = 0;
-  double rho = rho0;
-  while (niter < a.max_iter) {
-    niter++;
+  double* gc = st.gc;
+  while (st.niter < stop_at) {
+    st.niter++;
     const double s = sqrt(gc[3] / a.tlat);
     // moments in the frame of the current gauge: A = sum f, B = sum f xi, C = sum f |xi|^2
     float as = 0.f, ac = 0.f, bxs = 0.f, bxc = 0.f, bys = 0.f, byc = 0.f, bzs = 0.f, bzc = 0.f;
@@ -331,7 +324,8 @@ __global__ __launch_bounds__(64) void k_step_split27(const float* __restrict__ f
     // (sum f |v|^2/rho - |u|^2)/Dim = s^2 (C - |B|^2/rho) / (rho Dim).  This is the same
     // algebra without the cancellation of |u|^2 against sum f |v|^2 / rho, which at Mach 1
     // and above would amplify the FP32 rounding of the populations several times.
-    rho = (double)as - (double)ac;
+    const double rho = (double)as - (double)ac;
+    st.rho = rho;
     const double B[3] = {(double)bxs - (double)bxc, (double)bys - (double)byc, (double)bzs - (double)bzc};
     double m[4];
     if (rho != 0) {
@@ -345,11 +339,11 @@ __global__ __launch_bounds__(64) void k_step_split27(const float* __restrict__ f
     }
     if (m[3] < 0) m[3] = gc[3];
     // FP64 gauge residual, scaled by the thermal speed and the temperature
-    const double st = sqrt(fabs(m[3]));
+    const double sth = sqrt(fabs(m[3]));
     double res = 0;
     bool conv = true;
     for (int k = 0; k < 4; k++) {
-      const double scale = k < 3 ? st : fabs(m[3]);
+      const double scale = k < 3 ? sth : fabs(m[3]);
       const double d = fabs(m[k] - gc[k]);
       if (!(d <= a.tol_rel * scale + a.tol_abs)) conv = false;
       res = fmax(res, d / (scale + 1e-300));
@@ -357,24 +351,31 @@ __global__ __launch_bounds__(64) void k_step_split27(const float* __restrict__ f
     // FP32 populations leave a noise floor near the tolerance (interpolation rounding of a few
     // 1e-7 per population): a residual below 4 tol that has stopped falling has reached it.  A
     // residual that still falls, however slowly, keeps iterating down to tol.
-    if (!conv && niter > 1 && res <= 4.0 * a.tol_rel && res >= res_prev) conv = true;
+    if (!conv && st.niter > 1 && res <= 4.0 * a.tol_rel && res >= st.res_prev) conv = true;
     if (conv) {
       for (int k = 0; k < 4; k++) gc[k] = m[k];
-      break;
+      return true;
     }
-    if (learned && ((niter > 1 && !(res < res_prev)) || niter == kLearnedBudget)) {
+    if (st.learned && ((st.niter > 1 && !(res < st.res_prev)) || st.niter == kLearnedBudget)) {
       // the predicted start does not contract (or not within the budget): restart from the
       // previous gauge, so the cell ends where the plain method ends
       for (int k = 0; k < 4; k++) gc[k] = gp[k];
-      learned = false;
-      res_prev = 0;
+      st.learned = false;
+      st.res_prev = 0;
       continue;
     }
-    res_prev = res;
+    st.res_prev = res;
     for (int k = 0; k < 4; k++) gc[k] = m[k];
   }
-  // BGK in FP32 (steps.cu:557-564)
-  const float rf = (float)rho, dt = (float)a.dtau;
+  return false;
+}
+
+// BGK in FP32 (steps.cu:557-564) on the iterate left in fout, and the cell's outputs
+__device__ __forceinline__ void split27_finish(float* __restrict__ fout, float* __restrict__ mout,
+                                               double* __restrict__ Tout, int* __restrict__ niter_out, int64_t cell,
+                                               const SplitIter& st, const SplitArgs& a) {
+  const int64_t P = a.g.plane;
+  const float rf = (float)st.rho, dt = (float)a.dtau;
   for (int i = 0; i < 27; i++) {
     const int64_t o = (int64_t)i * P + cell;
     float f = fout[o];
@@ -382,13 +383,101 @@ __global__ __launch_bounds__(64) void k_step_split27(const float* __restrict__ f
     fout[o] = f;
   }
   mout[cell] = rf;
-  mout[P + cell] = (float)gc[0];
-  mout[2 * P + cell] = (float)gc[1];
-  mout[3 * P + cell] = (float)gc[2];
-  Tout[cell] = gc[3];
-  niter_out[cell] = niter;
+  mout[P + cell] = (float)st.gc[0];
+  mout[2 * P + cell] = (float)st.gc[1];
+  mout[3 * P + cell] = (float)st.gc[2];
+  Tout[cell] = st.gc[3];
+  niter_out[cell] = st.niter;
 }
 
+// First pass: every cell iterates up to a.pass1_iters times.  A wave runs as long as its
+// slowest cell, and on contact discontinuities and shocks a thin layer of cells needs several
+// times the mean, so cells still unconverged after the first pass are appended to a list
+// (wave-aggregated: one atomic per wave) and finished by k_step_split27_rest, whose waves
+// hold only such cells.  Each cell's arithmetic is the same as in a single pass.
+template <int INTERP>
+__global__ __launch_bounds__(64) void k_step_split27(const float* __restrict__ fin, const float* __restrict__ min,
+                                                     const double* __restrict__ Tin, const float* __restrict__ sqin,
+                                                     float* __restrict__ fout, float* __restrict__ mout,
+                                                     double* __restrict__ Tout, int* __restrict__ niter_out,
+                                                     const float* __restrict__ guess, SplitPending pend,
+                                                     const SplitArgs a) {
+  const Grid& g = a.g;
+  int x, y, z;
+  const bool live = tile_cell(g, xcd_block(blockIdx.x, gridDim.x), threadIdx.x, x, y, z);
+  bool pending = false;
+  SplitIter st;
+  const int64_t cell = live ? cell_index(g, x, y, z) : 0;
+  if (live) {
+    const int64_t P = g.plane;
+    double gp[4], rho0;
+    split_prev_gauge(min, Tin, P, cell, a.tlat, gp, rho0);
+    for (int k = 0; k < 4; k++) st.gc[k] = gp[k];
+    st.res_prev = 0;
+    st.rho = rho0;
+    st.niter = 0;
+    st.learned = false;
+    if (guess) {
+      // decode the predicted increment: (du/sqrt(T), dT/T) relative to the previous gauge
+      const double sq = sqrt(gp[3]);
+      const double q[4] = {gp[0] + sq * guess[cell], gp[1] + sq * guess[P + cell], gp[2] + sq * guess[2 * P + cell],
+                           gp[3] * (1.0 + (double)guess[3 * P + cell])};
+      if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0) {
+        for (int k = 0; k < 4; k++) st.gc[k] = q[k];
+        st.learned = true;
+      }
+    }
+    const NodeSrcF src = make_src(fin, min, sqin, g);
+    const int stop = a.pass1_iters < a.max_iter ? a.pass1_iters : a.max_iter;
+    const bool conv = split27_iterate<INTERP>(src, fout, cell, x, y, z, gp, st, stop, a);
+    pending = !conv && st.niter < a.max_iter;
+    if (!pending) split27_finish(fout, mout, Tout, niter_out, cell, st, a);
+  }
+  // append the pending cells of this wave to the list
+  bool any;
+  const int64_t e = pending_slot(pend, pending, any);
+  if (pending) {
+    pend.cell[e] = (int)cell;
+    for (int k = 0; k < 4; k++) pend.g[k * pend.cap + e] = st.gc[k];
+    pend.res[e] = st.res_prev;
+    pend.meta[e] = st.niter | (st.learned ? 0x10000 : 0);
+  }
+}
+
+// Second pass: a fixed grid of waves walks the pending list; every cell resumes its iteration
+// where the first pass left it.
+template <int INTERP>
+__global__ __launch_bounds__(64) void k_step_split27_rest(const float* __restrict__ fin, const float* __restrict__ min,
+                                                          const double* __restrict__ Tin, const float* __restrict__ sqin,
+                                                          float* __restrict__ fout, float* __restrict__ mout,
+                                                          double* __restrict__ Tout, int* __restrict__ niter_out,
+                                                          SplitPending pend, const SplitArgs a) {
+  const Grid& g = a.g;
+  const int count = *pend.count;
+  const NodeSrcF src = make_src(fin, min, sqin, g);
+  for (int64_t e = (int64_t)blockIdx.x * 64 + threadIdx.x; e - threadIdx.x < count; e += (int64_t)gridDim.x * 64) {
+    if (e >= count) continue;
+    const int64_t cell = pend.cell[e];
+    const int x = (int)(cell % g.nx);
+    const int y = (int)((cell / g.nx) % g.ny);
+    const int z = (int)(cell / ((int64_t)g.nx * g.ny));
+    double gp[4], rho0;
+    split_prev_gauge(min, Tin, g.plane, cell, a.tlat, gp, rho0);
+    SplitIter st;
+    for (int k = 0; k < 4; k++) st.gc[k] = pend.g[k * pend.cap + e];
+    st.res_prev = pend.res[e];
+    st.rho = rho0;
+    const int meta = pend.meta[e];
+    st.niter = meta & 0xffff;
+    st.learned = (meta & 0x10000) != 0;
+    split27_iterate<INTERP>(src, fout, cell, x, y, z, gp, st, a.max_iter, a);
+    split27_finish(fout, mout, Tout, niter_out, cell, st, a);
+  }
+}
+
+__global__ void k_zero_count(int* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0;
+}
 
 // ---------------------------------------------------------------------------------------------
 // D3Q125 split step: one 128-lane block per cell, lane = target population (PonDstepD3Q125,
@@ -597,7 +686,8 @@ hipError_t launch_sqrt_t(const double* T, float* sq, int64_t n, hipStream_t st) 
 }
 
 hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const SplitPtrs& out, const float* sq,
-                             int* niter, const float* guess, const SplitArgs& a, hipStream_t st) {
+                             int* niter, const float* guess, const SplitPending& pend, const SplitArgs& a,
+                             hipStream_t st) {
   if (lattice == kD3Q125) {
     hipLaunchKernelGGL(k_step_split125, dim3((unsigned)a.g.n), dim3(128), 0, st, in.f, in.m, in.T, out.f, out.m, out.T,
                        niter, guess, a);
@@ -605,9 +695,17 @@ hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const
   }
   if (lattice != kD3Q27) return hipErrorInvalidValue;
   const unsigned blocks = (unsigned)((int64_t)a.g.ntx * a.g.nty * a.g.ntz);
-#define POND_LAUNCH_S(I)                                                                                        \
-  hipLaunchKernelGGL((k_step_split27<I>), dim3(blocks), dim3(64), 0, st, in.f, in.m, in.T, sq, out.f, out.m, out.T, \
-                     niter, guess, a)
+  hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, st, pend.count);
+  // second-pass grid: enough waves to fill the chip several times, never more than the cells
+  const int64_t w2 = (a.g.n + 63) / 64;
+  const unsigned blocks2 = (unsigned)(w2 < 16384 ? w2 : 16384);
+#define POND_LAUNCH_S(I)                                                                                          \
+  do {                                                                                                            \
+    hipLaunchKernelGGL((k_step_split27<I>), dim3(blocks), dim3(64), 0, st, in.f, in.m, in.T, sq, out.f, out.m,     \
+                       out.T, niter, guess, pend, a);                                                             \
+    hipLaunchKernelGGL((k_step_split27_rest<I>), dim3(blocks2), dim3(64), 0, st, in.f, in.m, in.T, sq, out.f,     \
+                       out.m, out.T, niter, pend, a);                                                             \
+  } while (0)
   if (interp == 1) POND_LAUNCH_S(1);
   else if (interp == 2) POND_LAUNCH_S(2);
   else POND_LAUNCH_S(0);
