@@ -170,6 +170,7 @@ pond_status pond_mlp_init(pond_ctx* ctx, uint64_t seed) {
   if (!ctx) return POND_ERR_INVALID;
   pond_status s = mlp_alloc(ctx);
   if (s != POND_OK) return s;
+  MLP_HIP(ctx, hipStreamSynchronize(ctx->stream));  // no step may still read the network
   // He-uniform weights and zero biases from a seeded generator; training keeps the biases at zero
   std::vector<float> p(kMlpParams, 0.0f);
   std::mt19937_64 rng(seed);
@@ -246,7 +247,9 @@ pond_status pond_mlp_train(pond_ctx* ctx, int epochs, int batch, float lr, uint6
   MLP_HIP(ctx, hipMemsetAsync(m.mom, 0, 16 * sizeof(double), ctx->stream));
   MLP_HIP(ctx, mlp_moments2(m.X, m.Y, m.count, m.mom, ctx->stream));
   double mom[16];
-  MLP_HIP(ctx, hipMemcpy(mom, m.mom, 16 * sizeof(double), hipMemcpyDeviceToHost));
+  // the context stream is non-blocking: copies go through it, then wait for it
+  MLP_HIP(ctx, hipMemcpyAsync(mom, m.mom, 16 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  MLP_HIP(ctx, hipStreamSynchronize(ctx->stream));
   // the three velocity components share one scale (the lattice symmetries used in training
   // exchange them)
   double msi[5], mso[4];
@@ -299,7 +302,8 @@ pond_status pond_mlp_train(pond_ctx* ctx, int epochs, int batch, float lr, uint6
       if (st != POND_OK) break;
       if ((bi & 63) == 63 || bi == nb - 1) {
         float l = 0;
-        if (hipMemcpy(&l, m.loss, sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) {
+        if (hipMemcpyAsync(&l, m.loss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess) {
           st = POND_ERR_HIP;
           break;
         }
@@ -321,6 +325,7 @@ pond_status pond_mlp_get_params(pond_ctx* ctx, float* params, float* scales) {
   if (!ctx) return POND_ERR_INVALID;
   MlpCtx& m = ctx->mlp;
   if (!m.params) { ctx->err = "pond_mlp_get_params: no network"; return POND_ERR_STATE; }
+  MLP_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (params) MLP_HIP(ctx, hipMemcpy(params, m.params, (size_t)kMlpParams * 4, hipMemcpyDeviceToHost));
   if (scales) {
     for (int k = 0; k < 5; k++) scales[k] = m.in_inv[k];
@@ -334,6 +339,7 @@ pond_status pond_mlp_set_params(pond_ctx* ctx, const float* params, const float*
   pond_status s = mlp_alloc(ctx);
   if (s != POND_OK) return s;
   MlpCtx& m = ctx->mlp;
+  MLP_HIP(ctx, hipStreamSynchronize(ctx->stream));  // no step may still read the network
   MLP_HIP(ctx, hipMemcpy(m.params, params, (size_t)kMlpParams * 4, hipMemcpyHostToDevice));
   if (scales) {
     for (int k = 0; k < 5; k++) m.in_inv[k] = scales[k];

@@ -281,6 +281,12 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
   a.sqrt_tbase = std::sqrt(ctx->L.TLat);
   a.tlat = ctx->L.TLat;
   a.inv_dim = 1. / ctx->L.Dim;
+  // tuning switches: POND_XCD=0 launch-order workgroups, POND_NO_BUF=1 global loads (FP64)
+  g.xcd = 1;
+  if (const char* v = std::getenv("POND_XCD")) g.xcd = std::atoi(v) != 0;
+  a.g = g;
+  a.no_buf = 0;
+  if (const char* v = std::getenv("POND_NO_BUF")) a.no_buf = std::atoi(v) != 0;
   // iterations of the first pass of the two-pass step (POND_PASS1_ITERS overrides)
   a.pass1_iters = 3;
   if (const char* k = std::getenv("POND_PASS1_ITERS")) a.pass1_iters = std::atoi(k) > 0 ? std::atoi(k) : 3;
@@ -436,6 +442,9 @@ pond_status pond_set_populations(pond_ctx* ctx, const double* f, const double* r
                                  const double* uz, const double* T) {
   if (!ctx) return POND_ERR_INVALID;
   if (!f) return fail(ctx, POND_ERR_INVALID, "pond_set_populations: null f");
+  // the context stream is non-blocking and the copies below use the null stream: no queued
+  // step may still read or write the state buffers
+  POND_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const Grid& g = ctx->g;
   if (ctx->split) {
     const double* src[5] = {rho, ux, uy, uz, T};

@@ -17,6 +17,7 @@ struct Grid {
   // order, ntx*nty*ntz = ceil(n/64) waves
   int tx, ty, tz;
   int ntx, nty, ntz;
+  int xcd;  // 1: XCD-contiguous workgroup order (xcd_block), 0: launch order
 };
 
 // Parameters of one step, passed by value to the kernels.
@@ -29,6 +30,7 @@ struct StepArgs {
   double tlat;
   double inv_dim;     // 1./Dim (structs.cuh:330)
   int pass1_iters;    // iterations of the first pass of the one-lane-per-cell step
+  int no_buf;         // 1: global loads even where the buffer-resource loads apply
   double w[kQmax];    // lattice weights for the collide and set_base
 };
 
@@ -93,6 +95,7 @@ __device__ __forceinline__ int floor_int(double v) {
 // workgroups resident on one XCD overlap in its L2.  A bijection on [0, nblocks) for any count.
 __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks) {
   constexpr int kXcd = 8;
+  if (nblocks < 0) return b;
   const int64_t q = nblocks / kXcd, r = nblocks % kXcd;
   const int64_t x = b % kXcd, local = b / kXcd;
   return x * q + (x < r ? x : r) + local;

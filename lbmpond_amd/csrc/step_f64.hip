@@ -534,7 +534,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
   else src = gsrc;
   const Grid& g = a.g;
   int x, y, z;
-  const bool active = tile_cell(g, xcd_block(blockIdx.x, gridDim.x), threadIdx.x, x, y, z);
+  const bool active = tile_cell(g, xcd_block(blockIdx.x, g.xcd ? (int64_t)gridDim.x : -1), threadIdx.x, x, y, z);
   const int64_t cell = active ? cell_index(g, x, y, z) : 0;
   bool pending = false;
   F64Iter st;
@@ -604,7 +604,7 @@ __global__ __launch_bounds__(128) void k_step_q125_f64(const double* __restrict_
   __shared__ int sok[27];        // node populations finite and below 2^200
   const Grid& g = a.g;
   const int64_t P = g.plane;
-  const int64_t cell = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t cell = xcd_block(blockIdx.x, g.xcd ? (int64_t)gridDim.x : -1);
   const int k = threadIdx.x;
   const bool lane_ok = k < Q;
   const int x = (int)(cell % g.nx);
@@ -793,7 +793,7 @@ hipError_t launch_step_f64(int lattice, int interp, const double* fin, const dou
   const unsigned blocks = (unsigned)((int64_t)a.g.ntx * a.g.nty * a.g.ntz);
   const NodeSrc src{fin, min, Pin, okin, a.g};
   // buffer-resource loads while nine FP64 planes fit a 32-bit offset
-  const bool buf = (uint64_t)a.g.plane * 8u * kPlanesPerRsrc64 <= 0xffffffffull;
+  const bool buf = (uint64_t)a.g.plane * 8u * kPlanesPerRsrc64 <= 0xffffffffull && !a.no_buf;
   hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, st, pend.count);
   const int64_t w2 = (a.g.n + 63) / 64;
   const unsigned blocks2 = (unsigned)(w2 < 16384 ? w2 : 16384);

@@ -404,7 +404,7 @@ __global__ __launch_bounds__(64) void k_step_split27(const float* __restrict__ f
                                                      const SplitArgs a) {
   const Grid& g = a.g;
   int x, y, z;
-  const bool live = tile_cell(g, xcd_block(blockIdx.x, gridDim.x), threadIdx.x, x, y, z);
+  const bool live = tile_cell(g, xcd_block(blockIdx.x, g.xcd ? (int64_t)gridDim.x : -1), threadIdx.x, x, y, z);
   bool pending = false;
   SplitIter st;
   const int64_t cell = live ? cell_index(g, x, y, z) : 0;
@@ -517,7 +517,7 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
   __shared__ double red[10];    // per-wave partial sums of the five moments
   const Grid& g = a.g;
   const int64_t P = g.plane;
-  const int64_t cell = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t cell = xcd_block(blockIdx.x, g.xcd ? (int64_t)gridDim.x : -1);
   const int k = threadIdx.x;
   const bool lane_ok = k < Q;
   const int x = (int)(cell % g.nx);

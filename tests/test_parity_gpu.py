@@ -184,3 +184,22 @@ def test_nonfinite_and_absurd_gauges_stay_in_bounds(gpu):
             assert np.array_equal(np.isnan(got[k]), np.isnan(ref[k])), k
             fin = np.isfinite(ref[k])
             assert np.array_equal(got[k][fin], ref[k][fin]), k
+
+
+def test_two_pass_fp64_is_independent_of_first_pass_length(gpu, monkeypatch):
+    """The FP64 step's second pass resumes cells bit for bit: any first-pass length gives the
+    oracle's result."""
+    shape = (16, 12, 10)
+    f = smooth_random_state(shape, seed=71, T0=0.01)
+    o = po.Oracle("D3Q27", shape, "weno5c")
+    o.set_state(*f)
+    o.step(3)
+    ref = o.fields()
+    for k in ("1", "2"):
+        monkeypatch.setenv("POND_PASS1_ITERS", k)
+        with Pond("D3Q27", shape, "weno5c", "fp64") as p:
+            p.set_state(*f)
+            p.step(3)
+            got = p.fields()
+        for key in ("rho", "ux", "uy", "uz", "T", "niter"):
+            assert np.array_equal(got[key], ref[key]), (k, key)

@@ -120,3 +120,16 @@ def test_split_d3q125_turbulence(gpu):
     case = ic.isotropic_turbulence(n=12)
     errs, ref, got = compare(case.shape, "lagr4", case.fields, 10, 1e-5, dtau=case.dtau, lattice="D3Q125")
     assert got["niter_hist"].mean() <= ref["niter_hist"].mean()
+
+
+def test_two_pass_split_is_independent_of_first_pass_length(gpu, monkeypatch):
+    """Cells left unconverged by the first pass resume in the second with the same state: the
+    result is bitwise the same for any first-pass length (POND_PASS1_ITERS)."""
+    case = ic.shock_bubble(n=24)
+    outs = []
+    for k in ("1", "3", "100"):
+        monkeypatch.setenv("POND_PASS1_ITERS", k)
+        outs.append(run("D3Q27", case.shape, "weno5c", "split", case.fields, 3, dtau=case.dtau))
+    for o in outs[:2]:
+        for key in ("rho", "ux", "uy", "uz", "T", "niter"):
+            assert np.array_equal(o[key], outs[2][key]), key
