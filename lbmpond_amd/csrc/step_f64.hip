@@ -525,7 +525,7 @@ __device__ __forceinline__ void f64_finish(double* __restrict__ fout, double* __
 // arithmetic and operation order are those of a single pass.
 // BUF: node loads through buffer resources (planes up to 59,652,323 cells), else global loads.
 template <int LAT, int INTERP, bool BUF>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_step_f64(const NodeSrc gsrc, double* __restrict__ fout,
+__global__ __launch_bounds__(64) void k_step_f64(const NodeSrc gsrc, double* __restrict__ fout,
                                                  double* __restrict__ mout, int* __restrict__ niter_out, Pending pend,
                                                  const StepArgs a) {
   using Src = typename std::conditional<BUF, NodeBuf, NodeSrc>::type;
@@ -564,7 +564,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
 
 // Second pass: a fixed grid of waves walks the pending list, every cell resumes its iteration
 template <int LAT, int INTERP, bool BUF>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_step_f64_rest(const NodeSrc gsrc, double* __restrict__ fout,
+__global__ __launch_bounds__(64) void k_step_f64_rest(const NodeSrc gsrc, double* __restrict__ fout,
                                                       double* __restrict__ mout, int* __restrict__ niter_out,
                                                       Pending pend, const StepArgs a) {
   using Src = typename std::conditional<BUF, NodeBuf, NodeSrc>::type;
