@@ -147,7 +147,8 @@ def cpu_baseline(case_name, interp):
     return {"value": round(cells / dt / 1e6, 6), "unit": "MLUPS", "cores": pyoracle.threads(), "kind": "port",
             "sample": f"oracle (reference algorithm as written, no memoisation), {case.lattice} {interp} "
                       f"{sample_n}^3 cells x 1 step of the {case_name} initial-condition family "
-                      f"({cells} cell-steps in {dt:.1f} s, mean gauge iterations {niter:.2f})"}
+                      f"({cells} cell-steps in {dt:.1f} s, mean gauge iterations {niter:.2f}, "
+                      f"{cells * niter / dt / 1e6:.5f} M cell-iterations/s)"}
 
 
 def copy_calibration(nbytes):
@@ -256,7 +257,7 @@ def main():
     }
     if not args.no_cpu:
         log("[bench] cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(case.name, interp_for(case))
+        out["cpu_baseline"] = cpu_baseline(args.config, interp_for(case))
     print(json.dumps(out), flush=True)
 
 

@@ -292,7 +292,7 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
   // 128^3: K=3 2.98, K=5 3.19, K=8 3.35, single pass 2.86 MLUPS; split D3Q27, 192^3: K=3 7.6,
   // K=8 8.9, single pass 11.5 MLUPS (its second pass runs alone after the first and a few
   // slowly converging cells make a serial tail), so the split step runs in one pass.
-  a.pass1_iters = p->precision == POND_SPLIT ? p->max_iter : 8;
+  a.pass1_iters = p->precision == POND_SPLIT ? (1 << 30) : 8;
   if (const char* k = std::getenv("POND_PASS1_ITERS")) a.pass1_iters = std::atoi(k) > 0 ? std::atoi(k) : a.pass1_iters;
   for (int i = 0; i < ctx->Qn; i++) a.w[i] = ctx->L.w[i];
 
@@ -333,6 +333,8 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
     }
     SplitArgs& sa = ctx->sargs;
     sa.pass1_iters = a.pass1_iters;
+    sa.weno5c_variant = 0;
+    if (const char* v = std::getenv("POND_SPLIT_VARIANT")) sa.weno5c_variant = std::atoi(v);
     sa.g = g;
     sa.dtau = p->dtau;
     sa.tol_abs = p->split_tol_abs;

@@ -165,6 +165,32 @@ __device__ __forceinline__ void lag4f(float s, float F[4]) {
   F[3] = (a * b * c) * (1.0f / 6.0f);
 }
 
+// nonlinear WENO5 combination of the three sub-stencil values S at offset c = cx - r from the
+// five smoothness samples ub (steps.cu:795-825)
+__device__ __forceinline__ float weno5_combine(float c, const float ub[5], const float S[3]) {
+  const float d0 = (c * c - 3.f * c + 2.f) * (1.f / 12.f);
+  const float d2 = (c * c + 3.f * c + 2.f) * (1.f / 12.f);
+  const float d1 = 1.f - d0 - d2;
+  // Smoothness indicators in the difference form of Jiang & Shu (J. Comput. Phys. 126, 1996),
+  // which expands to exactly the reference's polynomials (steps.cu:801-806).  In FP32 the
+  // expanded form cancels terms of size ~25 f^2 down to beta ~ (df)^2, and the rounding of
+  // those terms, measured against the 1e-6 regularisation, moves the weights by ~1e-3 from one
+  // iteration to the next; the differences carry no such cancellation.
+  const float u1 = ub[0], u2 = ub[1], u3 = ub[2], u4 = ub[3], u5 = ub[4];
+  const float p0 = u1 - 2.f * u2 + u3, q0 = u1 - 4.f * u2 + 3.f * u3;
+  const float p1 = u2 - 2.f * u3 + u4, q1 = u2 - u4;
+  const float p2 = u3 - 2.f * u4 + u5, q2 = 3.f * u3 - 4.f * u4 + u5;
+  const float b0 = (13.f / 12.f) * p0 * p0 + 0.25f * q0 * q0;
+  const float b1 = (13.f / 12.f) * p1 * p1 + 0.25f * q1 * q1;
+  const float b2 = (13.f / 12.f) * p2 * p2 + 0.25f * q2 * q2;
+  const float e = 1e-6f;
+  const float w0 = d0 / ((e + b0) * (e + b0));
+  const float w1 = d1 / ((e + b1) * (e + b1));
+  const float w2 = d2 / ((e + b2) * (e + b2));
+  return (w0 * S[0] + w1 * S[1] + w2 * S[2]) / (w0 + w1 + w2);
+}
+
+
 // WENO5 interpolation (propagateWENO<3,5>) in FP32; CENTRED selects the corrected y/z stencil
 template <bool CENTRED>
 __device__ float interp_weno5f(const NodeSrcF& src, double cx, double cy, double cz, const TgtF& t, float sqTb) {
@@ -202,27 +228,8 @@ __device__ float interp_weno5f(const NodeSrcF& src, double cx, double cy, double
     if (xs >= 1 && xs <= 3) S1 = __builtin_fmaf(Fx[1][xs - 1], yzsum, S1);
     if (xs >= 2) S2 = __builtin_fmaf(Fx[2][xs - 2], yzsum, S2);
   }
-  const float c = (float)(cx - r);
-  const float d0 = (c * c - 3.f * c + 2.f) * (1.f / 12.f);
-  const float d2 = (c * c + 3.f * c + 2.f) * (1.f / 12.f);
-  const float d1 = 1.f - d0 - d2;
-  // Smoothness indicators in the difference form of Jiang & Shu (J. Comput. Phys. 126, 1996),
-  // which expands to exactly the reference's polynomials (steps.cu:801-806).  In FP32 the
-  // expanded form cancels terms of size ~25 f^2 down to beta ~ (df)^2, and the rounding of
-  // those terms, measured against the 1e-6 regularisation, moves the weights by ~1e-3 from one
-  // iteration to the next; the differences carry no such cancellation.
-  const float u1 = ub[0], u2 = ub[1], u3 = ub[2], u4 = ub[3], u5 = ub[4];
-  const float p0 = u1 - 2.f * u2 + u3, q0 = u1 - 4.f * u2 + 3.f * u3;
-  const float p1 = u2 - 2.f * u3 + u4, q1 = u2 - u4;
-  const float p2 = u3 - 2.f * u4 + u5, q2 = 3.f * u3 - 4.f * u4 + u5;
-  const float b0 = (13.f / 12.f) * p0 * p0 + 0.25f * q0 * q0;
-  const float b1 = (13.f / 12.f) * p1 * p1 + 0.25f * q1 * q1;
-  const float b2 = (13.f / 12.f) * p2 * p2 + 0.25f * q2 * q2;
-  const float e = 1e-6f;
-  const float w0 = d0 / ((e + b0) * (e + b0));
-  const float w1 = d1 / ((e + b1) * (e + b1));
-  const float w2 = d2 / ((e + b2) * (e + b2));
-  return (w0 * S0 + w1 * S1 + w2 * S2) / (w0 + w1 + w2);
+  const float Sv[3] = {S0, S1, S2};
+  return weno5_combine((float)(cx - r), ub, Sv);
 }
 
 __device__ float interp_lagr4f(const NodeSrcF& src, double cx, double cy, double cz, const TgtF& t, float sqTb) {
@@ -251,6 +258,120 @@ __device__ float interp_lagr4f(const NodeSrcF& src, double cx, double cy, double
   return out;
 }
 
+// WENO5C for the three targets of one (ey, ez) group: ex = -1, 0, +1.  They share the departure
+// rows in y and z (same y and z velocity), and their transfer bases differ only along x, so for
+// every stencil node the contraction of the 27 source populations over the z and y source
+// components, sy[jx] = sum_jy Ly[jy] sum_jz Lz[jz] f(jx,jy,jz), is common to the three:
+// 27 loads and 36 FMAs per node, then 3 FMAs and an x basis per target, where three separate
+// transfers load the node three times and contract it three times.  The x slices of the three
+// five-slice WENO windows are visited once each (their union, at most 5 + |r(-1) - r(+1)|).
+// Same interpolation as interp_weno5f<true> per target, with the y/z weights summed in a
+// different order.
+__device__ void interp_group_weno5c(const NodeSrcF& src, int x, int y, int z, double s, const double gc[4], int ey,
+                                    int ez, float sqTb, float out[3]) {
+  const Grid& g = src.g;
+  const float u1x = (float)gc[0], u1y = (float)gc[1], u1z = (float)gc[2];
+  const float u1xl = (float)(gc[0] - (double)u1x), u1yl = (float)(gc[1] - (double)u1y),
+              u1zl = (float)(gc[2] - (double)u1z);
+  const float T1 = (float)gc[3];
+  const float h = 1.0f / (2.0f * T1), mm = -1.0f / T1;
+  const float sq1 = sqrtf(fabsf(T1)) * (float)((0.0f < T1) - (T1 < 0.0f));
+  // y and z: shared rows, Lagrange weights and basis parameters
+  const double cy = (double)y - (s * (double)ey + gc[1]), cz = (double)z - (s * (double)ez + gc[2]);
+  const int ry = floor_int(cy + 0.5) - 1, rz = floor_int(cz + 0.5) - 1;
+  float Fy[3], Fz[3];
+  lag3f((float)(cy - ry), Fy);
+  lag3f((float)(cz - rz), Fz);
+  int yy[3], zz[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) { yy[a] = wrapi(ry + a, g.ny); zz[a] = wrapi(rz + a, g.nz); }
+  const bool ky = ey != 0, kz = ez != 0;
+  const float cyb = (float)ey * sq1, czb = (float)ez * sq1, wy = ey ? h : mm, wz = ez ? h : mm;
+  // x: per target t (ex = t - 1) its window r_t-2 .. r_t+2 and sub-stencil weights
+  int r[3];
+  float c[3];  // cx - r: sub-stencil k evaluates its quadratic at c + 2 - k
+#pragma unroll
+  for (int t = 0; t < 3; t++) {
+    const double cx = (double)x - (s * (double)(t - 1) + gc[0]);
+    r[t] = floor_int(cx + 0.5);
+    c[t] = (float)(cx - r[t]);
+  }
+  const int xlo = min(r[0], min(r[1], r[2])) - 2, xhi = max(r[0], max(r[1], r[2])) + 2;
+  float S[3][3], ub[3][5];
+#pragma unroll
+  for (int t = 0; t < 3; t++) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) S[t][k] = 0.f;
+#pragma unroll
+    for (int j = 0; j < 5; j++) ub[t][j] = 0.f;
+  }
+  for (int xs = xlo; xs <= xhi; xs++) {
+    const int xw = wrapi(xs, g.nx);
+    float yz[3] = {0.f, 0.f, 0.f}, mid[3] = {0.f, 0.f, 0.f};
+    // one node at a time: its 31 loads are the only ones in flight per lane
+#pragma unroll 1
+    for (int q = 0; q < 9; q++) {
+      const int a = q / 3, b = q - 3 * (q / 3);
+      const int ya = a == 0 ? yy[0] : a == 1 ? yy[1] : yy[2];
+      const int zb = b == 0 ? zz[0] : b == 1 ? zz[1] : zz[2];
+      const float fya = a == 0 ? Fy[0] : a == 1 ? Fy[1] : Fy[2];
+      const float fzb = b == 0 ? Fz[0] : b == 1 ? Fz[1] : Fz[2];
+      {
+        const uint32_t boff = (uint32_t)cell_index(g, xw, ya, zb) * 4u;
+        const float u0x = ldf(src.m, 1, boff), u0y = ldf(src.m, 2, boff), u0z = ldf(src.m, 3, boff);
+        const float sq0 = ldf(src.sq, 0, boff);
+        float Ly[3], Lz[3];
+        basis3(((u1y - u0y) + u1yl) * sqTb, sq0, ky, cyb, T1, wy, Ly);
+        basis3(((u1z - u0z) + u1zl) * sqTb, sq0, kz, czb, T1, wz, Lz);
+        float sy[3];
+#pragma unroll
+        for (int ja = 0; ja < 3; ja++) {
+          float acc = 0.f;
+#pragma unroll
+          for (int jb = 0; jb < 3; jb++) {
+            float sz = 0.f;
+#pragma unroll
+            for (int jc = 0; jc < 3; jc++) sz = __builtin_fmaf(Lz[jc], ldf(src.f, kIdx27.v[ja][jb][jc], boff), sz);
+            acc = __builtin_fmaf(Ly[jb], sz, acc);
+          }
+          sy[ja] = acc;
+        }
+        const float Dx = ((u1x - u0x) + u1xl) * sqTb;
+        const float wab = fya * fzb;
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+          float Lx[3];
+          basis3(Dx, sq0, t != 1, (float)(t - 1) * sq1, T1, t != 1 ? h : mm, Lx);
+          const float T = __builtin_fmaf(Lx[2], sy[2], __builtin_fmaf(Lx[1], sy[1], Lx[0] * sy[0]));
+          yz[t] = __builtin_fmaf(wab, T, yz[t]);
+          if (q == 4) mid[t] = T;
+        }
+      }
+    }
+    // slice xs enters target t's window at position j = xs - (r_t - 2)
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+      const int j = xs - (r[t] - 2);
+#pragma unroll
+      for (int jj = 0; jj < 5; jj++) {
+        if (j != jj) continue;
+        ub[t][jj] = mid[t];
+        // node jj - k of sub-stencil k (left node r-2+k), quadratic Lagrange weight at c + 2 - k
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          const int m = jj - k;
+          if (m < 0 || m > 2) continue;
+          const float sk = c[t] + (float)(2 - k);
+          const float wk = m == 0 ? 0.5f * (sk - 1.0f) * (sk - 2.0f) : m == 1 ? -sk * (sk - 2.0f) : 0.5f * sk * (sk - 1.0f);
+          S[t][k] = __builtin_fmaf(wk, yz[t], S[t][k]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 3; t++) out[t] = weno5_combine(c[t], ub[t], S[t]);
+}
+
 __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
   const float y = v - c;
   const float t = s + y;
@@ -276,7 +397,8 @@ struct SplitIter {
   double gc[4];      // current gauge (u, T)
   double res_prev;   // residual of the previous iteration
   double rho;        // density of the last iterate
-  int niter;
+  int niter;         // iterations so far, reported
+  int base;          // iterations spent on a predicted start that was abandoned
   bool learned;      // started from the predicted gauge and not yet fallen back
 };
 
@@ -288,36 +410,54 @@ __device__ __forceinline__ void split_prev_gauge(const float* __restrict__ min, 
   if (rho0 == 0 && gp[0] == 0 && gp[1] == 0 && gp[2] == 0 && gp[3] == 0) { rho0 = 1; gp[3] = tlat; }
 }
 
-// Fixed-point iterations of one cell from state st until it converges (true) or its count
-// reaches stop_at (false).  The iterate's populations are left in fout.
+// Fixed-point iterations of one cell from state st until it converges (true), its count
+// reaches stop_at, or its count since its current start reaches the cap (false).  A cell that
+// abandons a predicted start gets the full cap from its previous gauge, so a cell that does
+// not converge ends on the same iterate as without the prediction.  The iterate's populations
+// are left in fout.
 template <int INTERP>
 __device__ bool split27_iterate(const NodeSrcF& src, float* __restrict__ fout, int64_t cell, int x, int y, int z,
                                 const double gp[4], SplitIter& st, int stop_at, const SplitArgs& a) {
   const int64_t P = a.g.plane;
   const float sqTb = (float)a.sqrt_tbase;
   double* gc = st.gc;
-  while (st.niter < stop_at) {
+  while (st.niter < stop_at && st.niter - st.base < a.max_iter) {
     st.niter++;
     const double s = sqrt(gc[3] / a.tlat);
     // moments in the frame of the current gauge: A = sum f, B = sum f xi, C = sum f |xi|^2
     float as = 0.f, ac = 0.f, bxs = 0.f, bxc = 0.f, bys = 0.f, byc = 0.f, bzs = 0.f, bzc = 0.f;
     double cs = 0.0;
-    for (int i = 0; i < 27; i++) {
-      const int ex = kE27[i][0], ey = kE27[i][1], ez = kE27[i][2];
-      const double vx = s * (double)ex + gc[0];
-      const double vy = s * (double)ey + gc[1];
-      const double vz = s * (double)ez + gc[2];
-      const TgtF t = make_tgtf(i, gc);
-      const double cx = (double)x - vx, cy = (double)y - vy, cz = (double)z - vz;
-      const float fn = INTERP == 1   ? interp_lagr4f(src, cx, cy, cz, t, sqTb)
-                       : INTERP == 2 ? interp_weno5f<true>(src, cx, cy, cz, t, sqTb)
-                                     : interp_weno5f<false>(src, cx, cy, cz, t, sqTb);
-      fout[(int64_t)i * P + cell] = fn;
+    auto add = [&](int ex, int ey, int ez, float fn) {
+      fout[(int64_t)kIdx27.v[ex + 1][ey + 1][ez + 1] * P + cell] = fn;
       kahan_add(as, ac, fn);
       kahan_add(bxs, bxc, (float)ex * fn);
       kahan_add(bys, byc, (float)ey * fn);
       kahan_add(bzs, bzc, (float)ez * fn);
       cs += (double)fn * (double)(ex * ex + ey * ey + ez * ez);
+    };
+    if constexpr (INTERP == 2) {
+      // WENO5C: the three targets of each (ey, ez) group share their node loads and y/z work
+      for (int gi = 0; gi < 9; gi++) {
+        const int ey = gi / 3 - 1, ez = gi % 3 - 1;
+        float f3[3];
+        interp_group_weno5c(src, x, y, z, s, gc, ey, ez, sqTb, f3);
+        add(-1, ey, ez, f3[0]);
+        add(0, ey, ez, f3[1]);
+        add(1, ey, ez, f3[2]);
+      }
+    } else {
+      for (int i = 0; i < 27; i++) {
+        const int ex = kE27[i][0], ey = kE27[i][1], ez = kE27[i][2];
+        const double vx = s * (double)ex + gc[0];
+        const double vy = s * (double)ey + gc[1];
+        const double vz = s * (double)ez + gc[2];
+        const TgtF t = make_tgtf(i, gc);
+        const double cx = (double)x - vx, cy = (double)y - vy, cz = (double)z - vz;
+        const float fn = INTERP == 1   ? interp_lagr4f(src, cx, cy, cz, t, sqTb)
+                         : INTERP == 3 ? interp_weno5f<true>(src, cx, cy, cz, t, sqTb)
+                                       : interp_weno5f<false>(src, cx, cy, cz, t, sqTb);
+        add(ex, ey, ez, fn);
+      }
     }
     // FP64 moment solve (Cell::calcMoments, structs.cuh:315-342) restated in the gauge frame:
     // with v = u_g + s xi, sum f v / rho = u_g + s B/rho and
@@ -362,6 +502,7 @@ __device__ bool split27_iterate(const NodeSrcF& src, float* __restrict__ fout, i
       for (int k = 0; k < 4; k++) gc[k] = gp[k];
       st.learned = false;
       st.res_prev = 0;
+      st.base = st.niter;
       continue;
     }
     st.res_prev = res;
@@ -395,13 +536,15 @@ __device__ __forceinline__ void split27_finish(float* __restrict__ fout, float* 
 // times the mean, so cells still unconverged after the first pass are appended to a list
 // (wave-aggregated: one atomic per wave) and finished by k_step_split27_rest, whose waves
 // hold only such cells.  Each cell's arithmetic is the same as in a single pass.
+// INTERP: 0 WENO5 as written, 1 Lagrange-4, 2 WENO5C with the targets of an (ey, ez) group
+// sharing their node work, 3 WENO5C target by target.
 template <int INTERP>
-__global__ __launch_bounds__(64) void k_step_split27(const float* __restrict__ fin, const float* __restrict__ min,
-                                                     const double* __restrict__ Tin, const float* __restrict__ sqin,
-                                                     float* __restrict__ fout, float* __restrict__ mout,
-                                                     double* __restrict__ Tout, int* __restrict__ niter_out,
-                                                     const float* __restrict__ guess, SplitPending pend,
-                                                     const SplitArgs a) {
+__device__ __forceinline__ void split27_pass1(const float* __restrict__ fin, const float* __restrict__ min,
+                                              const double* __restrict__ Tin, const float* __restrict__ sqin,
+                                              float* __restrict__ fout, float* __restrict__ mout,
+                                              double* __restrict__ Tout, int* __restrict__ niter_out,
+                                              const float* __restrict__ guess, const SplitPending& pend,
+                                              const SplitArgs& a) {
   const Grid& g = a.g;
   int x, y, z;
   const bool live = tile_cell(g, xcd_block(blockIdx.x, g.xcd ? (int64_t)gridDim.x : -1), threadIdx.x, x, y, z);
@@ -416,6 +559,7 @@ __global__ __launch_bounds__(64) void k_step_split27(const float* __restrict__ f
     st.res_prev = 0;
     st.rho = rho0;
     st.niter = 0;
+    st.base = 0;
     st.learned = false;
     if (guess) {
       // decode the predicted increment: (du/sqrt(T), dT/T) relative to the previous gauge
@@ -428,9 +572,8 @@ __global__ __launch_bounds__(64) void k_step_split27(const float* __restrict__ f
       }
     }
     const NodeSrcF src = make_src(fin, min, sqin, g);
-    const int stop = a.pass1_iters < a.max_iter ? a.pass1_iters : a.max_iter;
-    const bool conv = split27_iterate<INTERP>(src, fout, cell, x, y, z, gp, st, stop, a);
-    pending = !conv && st.niter < a.max_iter;
+    const bool conv = split27_iterate<INTERP>(src, fout, cell, x, y, z, gp, st, a.pass1_iters, a);
+    pending = !conv && st.niter - st.base < a.max_iter;
     if (!pending) split27_finish(fout, mout, Tout, niter_out, cell, st, a);
   }
   // append the pending cells of this wave to the list
@@ -440,8 +583,26 @@ __global__ __launch_bounds__(64) void k_step_split27(const float* __restrict__ f
     pend.cell[e] = (int)cell;
     for (int k = 0; k < 4; k++) pend.g[k * pend.cap + e] = st.gc[k];
     pend.res[e] = st.res_prev;
-    pend.meta[e] = st.niter | (st.learned ? 0x10000 : 0);
+    pend.meta[e] = st.niter | (st.base << 8) | (st.learned ? 0x10000 : 0);
   }
+}
+
+template <int INTERP>
+__global__ __launch_bounds__(64) void k_step_split27(const float* __restrict__ fin, const float* __restrict__ min,
+                                                     const double* __restrict__ Tin, const float* __restrict__ sqin,
+                                                     float* __restrict__ fout, float* __restrict__ mout,
+                                                     double* __restrict__ Tout, int* __restrict__ niter_out,
+                                                     const float* __restrict__ guess, SplitPending pend,
+                                                     const SplitArgs a) {
+  split27_pass1<INTERP>(fin, min, Tin, sqin, fout, mout, Tout, niter_out, guess, pend, a);
+}
+
+// the grouped WENO5C step held to 168 VGPRs (3 waves per SIMD; a few spilled registers)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_step_split27_g3(
+    const float* __restrict__ fin, const float* __restrict__ min, const double* __restrict__ Tin,
+    const float* __restrict__ sqin, float* __restrict__ fout, float* __restrict__ mout, double* __restrict__ Tout,
+    int* __restrict__ niter_out, const float* __restrict__ guess, SplitPending pend, const SplitArgs a) {
+  split27_pass1<2>(fin, min, Tin, sqin, fout, mout, Tout, niter_out, guess, pend, a);
 }
 
 // Second pass: a fixed grid of waves walks the pending list; every cell resumes its iteration
@@ -468,9 +629,10 @@ __global__ __launch_bounds__(64) void k_step_split27_rest(const float* __restric
     st.res_prev = pend.res[e];
     st.rho = rho0;
     const int meta = pend.meta[e];
-    st.niter = meta & 0xffff;
+    st.niter = meta & 0xff;
+    st.base = (meta >> 8) & 0xff;
     st.learned = (meta & 0x10000) != 0;
-    split27_iterate<INTERP>(src, fout, cell, x, y, z, gp, st, a.max_iter, a);
+    split27_iterate<INTERP>(src, fout, cell, x, y, z, gp, st, 0x7fffffff, a);
     split27_finish(fout, mout, Tout, niter_out, cell, st, a);
   }
 }
@@ -552,10 +714,10 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
   const int X[3] = {kk / 25, (kk / 5) % 5, kk % 5};
   const double xa[3] = {node5(X[0], kEc1, kEc2), node5(X[1], kEc1, kEc2), node5(X[2], kEc1, kEc2)};
   const double xi2 = xa[0] * xa[0] + xa[1] * xa[1] + xa[2] * xa[2];
-  int niter = 0;
+  int niter = 0, base = 0;  // base: iterations spent on an abandoned predicted start
   double res_prev = 0, rho = rho0;
   float fn = 0.f;
-  while (niter < a.max_iter) {
+  while (niter - base < a.max_iter) {
     niter++;
     const double s = sqrt(gc[3] / a.tlat);
     // target basis constants (structs.cuh:397-416)
@@ -663,6 +825,7 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
       for (int d = 0; d < 4; d++) gc[d] = gp[d];
       learned = false;
       res_prev = 0;
+      base = niter;
       continue;
     }
     res_prev = res;
@@ -699,16 +862,18 @@ hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const
   // second-pass grid: enough waves to fill the chip several times, never more than the cells
   const int64_t w2 = (a.g.n + 63) / 64;
   const unsigned blocks2 = (unsigned)(w2 < 16384 ? w2 : 16384);
-#define POND_LAUNCH_S(I)                                                                                          \
+#define POND_LAUNCH_S(K, I)                                                                                       \
   do {                                                                                                            \
-    hipLaunchKernelGGL((k_step_split27<I>), dim3(blocks), dim3(64), 0, st, in.f, in.m, in.T, sq, out.f, out.m,     \
-                       out.T, niter, guess, pend, a);                                                             \
+    hipLaunchKernelGGL((K), dim3(blocks), dim3(64), 0, st, in.f, in.m, in.T, sq, out.f, out.m, out.T, niter, guess, \
+                       pend, a);                                                                                  \
     hipLaunchKernelGGL((k_step_split27_rest<I>), dim3(blocks2), dim3(64), 0, st, in.f, in.m, in.T, sq, out.f,     \
                        out.m, out.T, niter, pend, a);                                                             \
   } while (0)
-  if (interp == 1) POND_LAUNCH_S(1);
-  else if (interp == 2) POND_LAUNCH_S(2);
-  else POND_LAUNCH_S(0);
+  if (interp == 1) POND_LAUNCH_S(k_step_split27<1>, 1);
+  else if (interp == 2 && a.weno5c_variant == 1) POND_LAUNCH_S(k_step_split27_g3, 2);
+  else if (interp == 2 && a.weno5c_variant == 2) POND_LAUNCH_S(k_step_split27<3>, 3);
+  else if (interp == 2) POND_LAUNCH_S(k_step_split27<2>, 2);
+  else POND_LAUNCH_S(k_step_split27<0>, 0);
 #undef POND_LAUNCH_S
   return hipGetLastError();
 }
