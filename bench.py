@@ -74,13 +74,25 @@ def rel_linf(a, b):
     return float(np.abs(a - b).max() / den) if den > 0 else float(np.abs(a - b).max())
 
 
-def state_errors(got, ref):
+def state_errors(got, ref, mask=None):
     """relative L-inf of rho and T per field and of u as a vector (max component error over the
-    largest reference velocity component)"""
-    du = max(np.abs(got[k] - ref[k]).max() for k in ("ux", "uy", "uz"))
+    largest reference velocity component), over the cells where mask is true (all by default);
+    the normalisation always uses the whole reference field"""
+    sel = (lambda a: a) if mask is None else (lambda a: a[mask])
+    du = max(np.abs(sel(got[k]) - sel(ref[k])).max() for k in ("ux", "uy", "uz"))
     um = max(np.abs(ref[k]).max() for k in ("ux", "uy", "uz"))
-    return {"rho": rel_linf(got["rho"], ref["rho"]), "u": float(du / um) if um > 0 else float(du),
-            "T": rel_linf(got["T"], ref["T"])}
+    out = {"u": float(du / um) if um > 0 else float(du)}
+    for k in ("rho", "T"):
+        out[k] = float(np.abs(sel(got[k]) - sel(ref[k])).max() / np.abs(ref[k]).max())
+    return {k: out[k] for k in ("rho", "u", "T")}
+
+
+def cap_neighbourhood(capped, shape, reach=2):
+    """cells within `reach` (periodic, in each axis) of a cell that hit the iteration cap"""
+    from scipy.ndimage import maximum_filter
+    nx, ny, nz = shape
+    m = maximum_filter(capped.reshape(nz, ny, nx).astype(np.uint8), size=2 * reach + 1, mode="wrap")
+    return m.reshape(-1) > 0
 
 
 def run_path(case, precision, learned, collect, warmup, steps, train_cfg):
@@ -90,11 +102,13 @@ def run_path(case, precision, learned, collect, warmup, steps, train_cfg):
     p = Pond(case.lattice, case.shape, interp, precision, dtau=case.dtau, use_mlp=learned)
     p.set_state(*case.fields)
     res = {"path": tag}
+    capped = np.zeros(int(np.prod(case.shape)), bool)  # cells that hit the iteration cap in any step
     for c in range(collect):
         p.step(1)
         if learned:
             p.mlp_collect(train_cfg["samples_per_step"], seed=1000 + c)
         p.sync()
+        capped |= p.niter() >= p.params.max_iter
         log(f"[bench] {tag}: collect step {c + 1}/{collect}")
     if learned:
         t = time.perf_counter()
@@ -106,23 +120,29 @@ def run_path(case, precision, learned, collect, warmup, steps, train_cfg):
     for w in range(warmup):
         p.step(1)
         p.sync()
+        capped |= p.niter() >= p.params.max_iter
         log(f"[bench] {tag}: warmup step {w + 1}/{warmup}")
     t0 = p.timing()
     iters = []
     p.sync()
-    w0 = time.perf_counter()
+    wall = 0.0
     for k in range(steps):
+        # each timed step is bracketed by device synchronisation; the per-cell iteration counts
+        # are read between steps, outside the timed region
+        w0 = time.perf_counter()
         p.step(1)
         p.sync()
+        wall += time.perf_counter() - w0
         iters.append(p.diagnostics()["niter_mean"])
+        capped |= p.niter() >= p.params.max_iter
         log(f"[bench] {tag}: timed step {k + 1}/{steps}")
-    wall = time.perf_counter() - w0
     t1 = p.timing()
     cells = int(np.prod(case.shape))
     res.update(cells=cells, wall_s=wall, kernel_ms=t1["total_ms"] - t0["total_ms"], mlp_ms=t1["last_mlp_ms"],
                mlups=cells * steps / wall / 1e6, niter_mean=float(np.mean(iters)),
                niter_max=int(p.diagnostics()["niter_max"]))
     res["fields"] = p.fields()
+    res["capped"] = capped
     # a wave of 64 consecutive cells runs as many iterations as its slowest cell
     it = res["fields"]["niter"].astype(np.float64)
     pad = (-it.size) % 64
@@ -213,6 +233,13 @@ def main():
              "kernel_ms_per_step": round(r["kernel_ms"] / args.steps, 3)}
         if ref is not None and name != "fp64":
             s["rel_linf_vs_fp64"] = {k: float(f"{v:.3e}") for k, v in state_errors(r["fields"], ref["fields"]).items()}
+            # cells where either run hit the iteration cap in some step end on an unconverged
+            # iterate; the error over the cells away from those (5x5x5 neighbourhoods)
+            near = cap_neighbourhood(r["capped"] | ref["capped"], case.shape)
+            s["cells_near_cap"] = int(near.sum())
+            if near.any() and not near.all():
+                s["rel_linf_vs_fp64_away_from_cap"] = {
+                    k: float(f"{v:.3e}") for k, v in state_errors(r["fields"], ref["fields"], ~near).items()}
         for k in ("train_s", "train_samples", "train_loss"):
             if k in r:
                 s[k] = r[k]

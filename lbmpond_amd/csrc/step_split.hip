@@ -391,6 +391,10 @@ __global__ void k_sqrt_t(const double* __restrict__ T, float* __restrict__ sq, i
 // previous gauge.  Cells that need this many are the ill-conditioned ones (slow contraction at
 // contact discontinuities), whose converged state can depend on the starting point.
 constexpr int kLearnedBudget = 12;
+// Largest predicted change (in thermal units: du/sqrt(T), dT/T) used as a start.  Larger changes
+// sit on shocks and contact layers, where the gauge map can have separate fixed points and a
+// start far from the previous gauge may settle on another one than the reference reaches.
+constexpr float kLearnedTrust = 0.1f;
 
 // Iteration state of one cell, carried from the first pass to the second.
 struct SplitIter {
@@ -400,6 +404,7 @@ struct SplitIter {
   int niter;         // iterations so far, reported
   int base;          // iterations spent on a predicted start that was abandoned
   bool learned;      // started from the predicted gauge and not yet fallen back
+  bool contracted;   // a residual has fallen since the current start
 };
 
 // previous gauge of the cell (the plain start); an all-zero cell is reset as in steps.cu:360-363
@@ -491,7 +496,12 @@ __device__ bool split27_iterate(const NodeSrcF& src, float* __restrict__ fout, i
     // FP32 populations leave a noise floor near the tolerance (interpolation rounding of a few
     // 1e-7 per population): a residual below 4 tol that has stopped falling has reached it.  A
     // residual that still falls, however slowly, keeps iterating down to tol.
-    if (!conv && st.niter > 1 && res <= 4.0 * a.tol_rel && res >= st.res_prev) conv = true;
+    // (A predicted start must first show a falling residual: one that lands in the noise band
+    // at once may sit next to a fixed point the previous-gauge start does not reach.)
+    if (!conv && st.niter - st.base > 1 && res <= 4.0 * a.tol_rel && res >= st.res_prev &&
+        (!st.learned || st.contracted))
+      conv = true;
+    if (st.niter - st.base > 1 && res < st.res_prev) st.contracted = true;
     if (conv) {
       for (int k = 0; k < 4; k++) gc[k] = m[k];
       return true;
@@ -501,6 +511,7 @@ __device__ bool split27_iterate(const NodeSrcF& src, float* __restrict__ fout, i
       // previous gauge, so the cell ends where the plain method ends
       for (int k = 0; k < 4; k++) gc[k] = gp[k];
       st.learned = false;
+      st.contracted = false;
       st.res_prev = 0;
       st.base = st.niter;
       continue;
@@ -561,12 +572,14 @@ __device__ __forceinline__ void split27_pass1(const float* __restrict__ fin, con
     st.niter = 0;
     st.base = 0;
     st.learned = false;
+    st.contracted = false;
     if (guess) {
       // decode the predicted increment: (du/sqrt(T), dT/T) relative to the previous gauge
       const double sq = sqrt(gp[3]);
-      const double q[4] = {gp[0] + sq * guess[cell], gp[1] + sq * guess[P + cell], gp[2] + sq * guess[2 * P + cell],
-                           gp[3] * (1.0 + (double)guess[3 * P + cell])};
-      if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0) {
+      const float p0 = guess[cell], p1 = guess[P + cell], p2 = guess[2 * P + cell], p3 = guess[3 * P + cell];
+      const double q[4] = {gp[0] + sq * p0, gp[1] + sq * p1, gp[2] + sq * p2, gp[3] * (1.0 + (double)p3)};
+      const float pmax = fmaxf(fmaxf(fabsf(p0), fabsf(p1)), fmaxf(fabsf(p2), fabsf(p3)));
+      if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0 && pmax <= kLearnedTrust) {
         for (int k = 0; k < 4; k++) st.gc[k] = q[k];
         st.learned = true;
       }
@@ -583,7 +596,7 @@ __device__ __forceinline__ void split27_pass1(const float* __restrict__ fin, con
     pend.cell[e] = (int)cell;
     for (int k = 0; k < 4; k++) pend.g[k * pend.cap + e] = st.gc[k];
     pend.res[e] = st.res_prev;
-    pend.meta[e] = st.niter | (st.base << 8) | (st.learned ? 0x10000 : 0);
+    pend.meta[e] = st.niter | (st.base << 8) | (st.learned ? 0x10000 : 0) | (st.contracted ? 0x20000 : 0);
   }
 }
 
@@ -632,6 +645,7 @@ __global__ __launch_bounds__(64) void k_step_split27_rest(const float* __restric
     st.niter = meta & 0xff;
     st.base = (meta >> 8) & 0xff;
     st.learned = (meta & 0x10000) != 0;
+    st.contracted = (meta & 0x20000) != 0;
     split27_iterate<INTERP>(src, fout, cell, x, y, z, gp, st, 0x7fffffff, a);
     split27_finish(fout, mout, Tout, niter_out, cell, st, a);
   }
@@ -704,7 +718,9 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
     const double st = sqrt(gp[3]);
     const double q[4] = {gp[0] + st * guess[cell], gp[1] + st * guess[P + cell], gp[2] + st * guess[2 * P + cell],
                          gp[3] * (1.0 + (double)guess[3 * P + cell])};
-    if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0) {
+    const float pmax = fmaxf(fmaxf(fabsf(guess[cell]), fabsf(guess[P + cell])),
+                             fmaxf(fabsf(guess[2 * P + cell]), fabsf(guess[3 * P + cell])));
+    if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0 && pmax <= kLearnedTrust) {
       for (int d = 0; d < 4; d++) gc[d] = q[d];
       learned = true;
     }
@@ -715,6 +731,7 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
   const double xa[3] = {node5(X[0], kEc1, kEc2), node5(X[1], kEc1, kEc2), node5(X[2], kEc1, kEc2)};
   const double xi2 = xa[0] * xa[0] + xa[1] * xa[1] + xa[2] * xa[2];
   int niter = 0, base = 0;  // base: iterations spent on an abandoned predicted start
+  bool contracted = false;   // a residual has fallen since the current start
   double res_prev = 0, rho = rho0;
   float fn = 0.f;
   while (niter - base < a.max_iter) {
@@ -816,7 +833,8 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
       if (!(dd <= a.tol_rel * scale + a.tol_abs)) conv = false;
       res = fmax(res, dd / (scale + 1e-300));
     }
-    if (!conv && niter > 1 && res <= 4.0 * a.tol_rel && res >= res_prev) conv = true;
+    if (!conv && niter - base > 1 && res <= 4.0 * a.tol_rel && res >= res_prev && (!learned || contracted)) conv = true;
+    if (niter - base > 1 && res < res_prev) contracted = true;
     if (conv) {
       for (int d = 0; d < 4; d++) gc[d] = m[d];
       break;
@@ -824,6 +842,7 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
     if (learned && ((niter > 1 && !(res < res_prev)) || niter == kLearnedBudget)) {
       for (int d = 0; d < 4; d++) gc[d] = gp[d];
       learned = false;
+      contracted = false;
       res_prev = 0;
       base = niter;
       continue;

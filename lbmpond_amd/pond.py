@@ -257,6 +257,13 @@ class Pond:
                                           out["niter"].ctypes.data_as(_I32)), "pond_get_fields")
         return out
 
+    def niter(self):
+        """gauge iterations of every cell in the last step (pond_get_fields with only niter)"""
+        it = np.zeros(self.n, np.int32)
+        self._check(lib().pond_get_fields(self._h, None, None, None, None, None, it.ctypes.data_as(_I32)),
+                    "pond_get_fields")
+        return it
+
     def populations(self):
         f = np.zeros(self.n * self.qn)
         self._check(lib().pond_get_populations(self._h, _dptr(f)), "pond_get_populations")
