@@ -172,6 +172,13 @@ pond_status pond_mlp_set_params(pond_ctx* ctx, const float* params, const float*
    scales applied), through the fused MFMA inference kernel */
 pond_status pond_mlp_forward(pond_ctx* ctx, const float* X, int64_t B, float* Y);
 
+/* the network's predicted start for every cell of the current state, as the step uses it:
+   out[4][n] (x fastest) = (du_x/sqrt(T), du_y/sqrt(T), du_z/sqrt(T), dT/T) relative to the
+   cell's gauge, computed by the fused inference kernel from the 3x3x3 neighbourhoods.  No
+   counterpart in the reference (the learned initializer is new); for inspecting the learned
+   start. */
+pond_status pond_mlp_predict(pond_ctx* ctx, float* out);
+
 /* gradient of the mean-squared loss of a host batch (X[B][135], targets T[B][4], B a multiple
    of 16) through the MFMA training kernels; grad has pond_mlp_num_params() entries */
 pond_status pond_mlp_grad(pond_ctx* ctx, const float* X, const float* T, int64_t B, float* grad, float* loss);

@@ -56,6 +56,16 @@ __device__ __forceinline__ float raw_feature(const MlpState& s, int x, int y, in
 
 // ------------------------------------------------------------------------ fused inference
 
+// cells of the neighbourhood of a 16-cell row tile: 18 x 3 x 3
+constexpr int kStage = 18 * 9;
+
+// orders this wave's LDS writes before its following reads (waves stage independently)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // One wave per 16-cell tile loop.  Features are taken from the state (mode 0) or from a
 // normalized feature matrix (mode 1, for checking the network arithmetic).
 template <int MODE>
@@ -101,30 +111,43 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const MlpWeightsBF w, MlpStat
 #pragma unroll
   for (int e = 0; e < 4; e++) b3[e] = (4 * q + e) < kMlpOut ? w.b3[4 * q + e] : 0.0f;
 
-  const int64_t ntile = (ncell + 15) / 16;
   const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int64_t nwave = (int64_t)gridDim.x * (blockDim.x >> 6);
+  // MODE 0: tiles are 16 consecutive cells of one x-row (a row has ceil(nx/16) tiles), so the
+  // 3x3x3 neighbourhoods of a tile's cells lie in 18 x 3 x 3 cells; the wave stages their
+  // (rho, u, T) in LDS once and builds the 135 features of every cell from there.
+  __shared__ float lm[4][kStage][4];
+  __shared__ double lT[4][kStage];
+  const int wv = threadIdx.x >> 6;
+  const Grid& g = s.g;
+  const int ntr = (g.nx + 15) / 16;
+  const int64_t ntile = MODE == 0 ? (int64_t)ntr * g.ny * g.nz : (ncell + 15) / 16;
   for (int64_t tile = wave; tile < ntile; tile += nwave) {
-    const int64_t cell = tile * 16 + col;
-    const bool live = cell < ncell;
+    int64_t cell;
+    bool live;
     // B operands of layer 1: features 16*st + 4*q + e of this lane's cell
     short4_t xb[9];
     if (MODE == 0) {
-      int x = 0, y = 0, z = 0;
-      float rc = 1, uxc = 0, uyc = 0, uzc = 0, isT = 1;
-      double Tc = 1;
-      if (live) {
-        const Grid& g = s.g;
-        x = (int)(cell % g.nx);
-        y = (int)((cell / g.nx) % g.ny);
-        z = (int)(cell / ((int64_t)g.nx * g.ny));
-        rc = s.m[cell];
-        uxc = s.m[g.plane + cell];
-        uyc = s.m[2 * g.plane + cell];
-        uzc = s.m[3 * g.plane + cell];
-        Tc = s.T[cell];
-        isT = (float)(1.0 / sqrt(Tc));
+      const int xt = (int)(tile % ntr);
+      const int64_t row = tile / ntr;
+      const int y = (int)(row % g.ny), z = (int)(row / g.ny);
+      const int x = 16 * xt + col;
+      live = x < g.nx;
+      cell = cell_index(g, live ? x : 0, y, z);
+      wave_sync();  // the previous tile's reads of this wave's LDS are done
+      for (int e = l; e < kStage; e += 64) {
+        const int xx = e % 18, r = e / 18;
+        const int64_t c =
+            cell_index(g, wrapi(16 * xt - 1 + xx, g.nx), wrapi(y + r / 3 - 1, g.ny), wrapi(z + r % 3 - 1, g.nz));
+#pragma unroll
+        for (int k = 0; k < 4; k++) lm[wv][e][k] = s.m[(int64_t)k * g.plane + c];
+        lT[wv][e] = s.T[c];
       }
+      wave_sync();
+      const int ec = 4 * 18 + col + 1;  // the cell itself: dx = dy = dz = 0
+      const float rc = lm[wv][ec][0], uxc = lm[wv][ec][1], uyc = lm[wv][ec][2], uzc = lm[wv][ec][3];
+      const double Tc = lT[wv][ec];
+      const float isT = (float)(1.0 / sqrt(Tc));
 #pragma unroll
       for (int st = 0; st < 9; st++) {
         short4_t v;
@@ -132,12 +155,28 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const MlpWeightsBF w, MlpStat
         for (int e = 0; e < 4; e++) {
           const int f = 16 * st + 4 * q + e;
           float val = 0.0f;
-          if (live && f < kMlpIn) val = raw_feature(s, x, y, z, f, rc, uxc, uyc, uzc, Tc, isT) * w.in_inv[f % 5];
+          if (live && f < kMlpIn) {
+            // raw_feature, read from the staged neighbourhood
+            const int nb = f / 5, type = f - 5 * (f / 5);
+            const int dx = nb / 9 - 1, dyz = nb - 9 * (nb / 9);  // (dy+1)*3 + (dz+1)
+            const int en = dyz * 18 + col + 1 + dx;
+            float raw;
+            switch (type) {
+              case 0: raw = lm[wv][en][0] / rc - 1.0f; break;
+              case 1: raw = (lm[wv][en][1] - uxc) * isT; break;
+              case 2: raw = (lm[wv][en][2] - uyc) * isT; break;
+              case 3: raw = (lm[wv][en][3] - uzc) * isT; break;
+              default: raw = (float)(lT[wv][en] / Tc - 1.0); break;
+            }
+            val = raw * w.in_inv[type];
+          }
           v[e] = (short)f2bf(val);
         }
         xb[st] = v;
       }
     } else {
+      cell = tile * 16 + col;
+      live = cell < ncell;
 #pragma unroll
       for (int st = 0; st < 9; st++) {
         short4_t v;
@@ -425,7 +464,7 @@ __global__ void k_moments2(const float* __restrict__ X, const float* __restrict_
 // ------------------------------------------------------------------------ launchers
 
 hipError_t mlp_launch_infer(const MlpWeightsBF& w, const MlpState& s, float* out, int64_t out_plane, hipStream_t st) {
-  const int64_t ntile = (s.g.n + 15) / 16;
+  const int64_t ntile = (int64_t)((s.g.nx + 15) / 16) * s.g.ny * s.g.nz;
   const int64_t waves = ntile < 16384 ? ntile : 16384;
   const unsigned blocks = (unsigned)((waves + 3) / 4);
   hipLaunchKernelGGL(k_mlp_infer<0>, dim3(blocks), dim3(256), 0, st, w, s, (const float*)nullptr, s.g.n, out, out_plane);

@@ -356,6 +356,26 @@ pond_status pond_mlp_set_params(pond_ctx* ctx, const float* params, const float*
   return POND_OK;
 }
 
+pond_status pond_mlp_predict(pond_ctx* ctx, float* out) {
+  if (!ctx || !out) return POND_ERR_INVALID;
+  MlpCtx& m = ctx->mlp;
+  if (!m.params) { ctx->err = "pond_mlp_predict: no network"; return POND_ERR_STATE; }
+  if (!ctx->split || !ctx->has_state) { ctx->err = "pond_mlp_predict: needs a POND_SPLIT context with a state"; return POND_ERR_STATE; }
+  const Grid& g = ctx->g;
+  float* d = nullptr;
+  MLP_HIP(ctx, hipMalloc(&d, (size_t)4 * g.plane * sizeof(float)));
+  hipError_t e = mlp_launch_infer(pond_mlp_weights(ctx), state_view(ctx, ctx->cur), d, g.plane, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  for (int k = 0; k < 4 && e == hipSuccess; k++)
+    e = hipMemcpy(out + (size_t)k * g.n, d + (size_t)k * g.plane, (size_t)g.n * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) {
+    ctx->err = std::string("pond_mlp_predict: ") + hipGetErrorString(e);
+    return POND_ERR_HIP;
+  }
+  return POND_OK;
+}
+
 pond_status pond_mlp_forward(pond_ctx* ctx, const float* X, int64_t B, float* Y) {
   if (!ctx || !X || !Y || B < 1) return POND_ERR_INVALID;
   MlpCtx& m = ctx->mlp;

@@ -112,6 +112,7 @@ SIGNATURES = {
     "pond_mlp_get_params": (ctypes.c_int, [_CTX, _F, _F]),
     "pond_mlp_set_params": (ctypes.c_int, [_CTX, _F, _F]),
     "pond_mlp_forward": (ctypes.c_int, [_CTX, _F, ctypes.c_int64, _F]),
+    "pond_mlp_predict": (ctypes.c_int, [_CTX, _F]),
     "pond_mlp_grad": (ctypes.c_int, [_CTX, _F, _F, ctypes.c_int64, _F, _F]),
 }
 
@@ -324,6 +325,12 @@ class Pond:
         self._check(lib().pond_mlp_forward(self._h, X.ctypes.data_as(_F), B, Y.ctypes.data_as(_F)),
                     "pond_mlp_forward")
         return Y
+
+    def mlp_predict(self):
+        """predicted start of every cell from the current state: (4, n) float32"""
+        out = np.zeros(4 * self.n, np.float32)
+        self._check(lib().pond_mlp_predict(self._h, out.ctypes.data_as(_F)), "pond_mlp_predict")
+        return out.reshape(4, self.n)
 
     def mlp_grad(self, X, T):
         X = np.ascontiguousarray(X, np.float32)

@@ -96,3 +96,37 @@ def test_uniform_neighbourhood_keeps_plain_start(gpu):
             outs.append(p.fields())
     for k in FIELDS + ("niter",):
         assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+def test_fused_inference_on_state_matches_oracle(gpu):
+    """The step's fused inference (features gathered from the staged 3x3x3 neighbourhoods of
+    16-cell row tiles) against the oracle network on features built on the host, on a grid
+    whose rows are not a multiple of the tile."""
+    from helpers import smooth_random_state
+    shape = (20, 7, 5)
+    nx, ny, nz = shape
+    params, _, _ = make(13, B=16)
+    scales = np.array([3.0, 4.0, 4.0, 4.0, 5.0, 0.2, 0.2, 0.2, 0.1], np.float32)
+    fields = smooth_random_state(shape, seed=81, T0=0.02)
+    with Pond("D3Q27", shape, "weno5c", "split", use_mlp=True) as p:
+        p.set_state(*fields)
+        p.mlp_set_params(params, scales)
+        st = p.fields()
+        got = p.mlp_predict()
+    rho, ux, uy, uz = (st[k].astype(np.float32).reshape(nz, ny, nx) for k in ("rho", "ux", "uy", "uz"))
+    T = st["T"].reshape(nz, ny, nx)
+    isT = (1.0 / np.sqrt(T)).astype(np.float32)
+    F = np.zeros((nx * ny * nz, 135), np.float32)
+    for nb in range(27):
+        dx, dy, dz = nb // 9 - 1, (nb // 3) % 3 - 1, nb % 3 - 1
+        sh = lambda a: np.roll(a, shift=(-dz, -dy, -dx), axis=(0, 1, 2))
+        F[:, 5 * nb + 0] = (sh(rho) / rho - np.float32(1)).reshape(-1)
+        F[:, 5 * nb + 1] = ((sh(ux) - ux) * isT).reshape(-1)
+        F[:, 5 * nb + 2] = ((sh(uy) - uy) * isT).reshape(-1)
+        F[:, 5 * nb + 3] = ((sh(uz) - uz) * isT).reshape(-1)
+        F[:, 5 * nb + 4] = (sh(T) / T - 1.0).astype(np.float32).reshape(-1)
+    Xn = (F.reshape(-1, 27, 5) * scales[:5]).reshape(-1, 135).astype(np.float32)
+    ref = po.mlp_forward(params, Xn, bf16=True) * scales[5:]
+    err = np.abs(got.T - ref).max() / np.abs(ref).max()
+    print("fused inference rel Linf vs oracle:", err)
+    assert err < 2e-3
