@@ -31,6 +31,8 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
+from lbmpond_amd.compare import CapDependence, state_errors  # noqa: E402
+
 METRIC = "MLUPS + mean gauge iters/step, learned+FP32 vs plain FP64, at stated L∞ err"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
@@ -69,32 +71,6 @@ def interp_for(case):
     return "weno5c" if case.lattice == "D3Q27" else case.interp
 
 
-def rel_linf(a, b):
-    den = np.abs(b).max()
-    return float(np.abs(a - b).max() / den) if den > 0 else float(np.abs(a - b).max())
-
-
-def state_errors(got, ref, mask=None):
-    """relative L-inf of rho and T per field and of u as a vector (max component error over the
-    largest reference velocity component), over the cells where mask is true (all by default);
-    the normalisation always uses the whole reference field"""
-    sel = (lambda a: a) if mask is None else (lambda a: a[mask])
-    du = max(np.abs(sel(got[k]) - sel(ref[k])).max() for k in ("ux", "uy", "uz"))
-    um = max(np.abs(ref[k]).max() for k in ("ux", "uy", "uz"))
-    out = {"u": float(du / um) if um > 0 else float(du)}
-    for k in ("rho", "T"):
-        out[k] = float(np.abs(sel(got[k]) - sel(ref[k])).max() / np.abs(ref[k]).max())
-    return {k: out[k] for k in ("rho", "u", "T")}
-
-
-def cap_neighbourhood(capped, shape, reach=2):
-    """cells within `reach` (periodic, in each axis) of a cell that hit the iteration cap"""
-    from scipy.ndimage import maximum_filter
-    nx, ny, nz = shape
-    m = maximum_filter(capped.reshape(nz, ny, nx).astype(np.uint8), size=2 * reach + 1, mode="wrap")
-    return m.reshape(-1) > 0
-
-
 def run_path(case, precision, learned, collect, warmup, steps, train_cfg):
     from lbmpond_amd import Pond
     interp = interp_for(case)
@@ -102,13 +78,13 @@ def run_path(case, precision, learned, collect, warmup, steps, train_cfg):
     p = Pond(case.lattice, case.shape, interp, precision, dtau=case.dtau, use_mlp=learned)
     p.set_state(*case.fields)
     res = {"path": tag}
-    capped = np.zeros(int(np.prod(case.shape)), bool)  # cells that hit the iteration cap in any step
+    dep = CapDependence(case.shape, p.params.max_iter)  # cells downstream of an iteration-capped cell
     for c in range(collect):
         p.step(1)
         if learned:
             p.mlp_collect(train_cfg["samples_per_step"], seed=1000 + c)
         p.sync()
-        capped |= p.niter() >= p.params.max_iter
+        dep.update(p.niter())
         log(f"[bench] {tag}: collect step {c + 1}/{collect}")
     if learned:
         t = time.perf_counter()
@@ -120,7 +96,7 @@ def run_path(case, precision, learned, collect, warmup, steps, train_cfg):
     for w in range(warmup):
         p.step(1)
         p.sync()
-        capped |= p.niter() >= p.params.max_iter
+        dep.update(p.niter())
         log(f"[bench] {tag}: warmup step {w + 1}/{warmup}")
     t0 = p.timing()
     iters = []
@@ -134,7 +110,7 @@ def run_path(case, precision, learned, collect, warmup, steps, train_cfg):
         p.sync()
         wall += time.perf_counter() - w0
         iters.append(p.diagnostics()["niter_mean"])
-        capped |= p.niter() >= p.params.max_iter
+        dep.update(p.niter())
         log(f"[bench] {tag}: timed step {k + 1}/{steps}")
     t1 = p.timing()
     cells = int(np.prod(case.shape))
@@ -142,7 +118,7 @@ def run_path(case, precision, learned, collect, warmup, steps, train_cfg):
                mlups=cells * steps / wall / 1e6, niter_mean=float(np.mean(iters)),
                niter_max=int(p.diagnostics()["niter_max"]))
     res["fields"] = p.fields()
-    res["capped"] = capped
+    res["cap_dependence"] = dep
     # a wave of 64 consecutive cells runs as many iterations as its slowest cell
     it = res["fields"]["niter"].astype(np.float64)
     pad = (-it.size) % 64
@@ -155,7 +131,7 @@ def cpu_baseline(case_name, interp):
     """oracle on a bounded sample of the same IC family, all host threads"""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    sample_n = {"c2": 24, "c3": 24, "c4": 24, "c5": 8}[case_name]
+    sample_n = {"c2": 32, "c3": 32, "c4": 32, "c5": 8}[case_name]
     case = make_case(case_name, sample_n)
     o = pyoracle.Oracle(case.lattice, case.shape, interp, dtau=case.dtau)
     o.set_state(*case.fields)
@@ -169,6 +145,23 @@ def cpu_baseline(case_name, interp):
                       f"{sample_n}^3 cells x 1 step of the {case_name} initial-condition family "
                       f"({cells} cell-steps in {dt:.1f} s, mean gauge iterations {niter:.2f}, "
                       f"{cells * niter / dt / 1e6:.5f} M cell-iterations/s)"}
+
+
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3 counter
+    summaries: a line 'traffic: kernel=K workload=W bytes_per_launch=B' in profiles/*.md (FETCH_SIZE
+    and WRITE_SIZE per dispatch, scaled by the same counters' reading of a plain device copy)."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as fh:
+            for line in fh:
+                line = line.strip().strip("`")
+                if not line.startswith("traffic:"):
+                    continue
+                kv = dict(t.split("=", 1) for t in line[len("traffic:"):].split() if "=" in t)
+                if kv.get("kernel") == kernel and kv.get("workload") == workload:
+                    return float(kv["bytes_per_launch"]), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def copy_calibration(nbytes):
@@ -233,13 +226,15 @@ def main():
              "kernel_ms_per_step": round(r["kernel_ms"] / args.steps, 3)}
         if ref is not None and name != "fp64":
             s["rel_linf_vs_fp64"] = {k: float(f"{v:.3e}") for k, v in state_errors(r["fields"], ref["fields"]).items()}
-            # cells where either run hit the iteration cap in some step end on an unconverged
-            # iterate; the error over the cells away from those (5x5x5 neighbourhoods)
-            near = cap_neighbourhood(r["capped"] | ref["capped"], case.shape)
-            s["cells_near_cap"] = int(near.sum())
-            if near.any() and not near.all():
-                s["rel_linf_vs_fp64_away_from_cap"] = {
-                    k: float(f"{v:.3e}") for k, v in state_errors(r["fields"], ref["fields"], ~near).items()}
+            # a cell that hit the iteration cap in either run ends its step on an unconverged
+            # iterate that depends on where its loop started; the error over the cells outside
+            # the region such cells can have influenced (3 cells per step)
+            dep = r["cap_dependence"].mask | ref["cap_dependence"].mask
+            s["capped_cells_per_step"] = {"fp64": ref["cap_dependence"].capped, name: r["cap_dependence"].capped}
+            s["cells_in_cap_dependence"] = int(dep.sum())
+            if not dep.all():
+                s["rel_linf_vs_fp64_outside_cap_dependence"] = {
+                    k: float(f"{v:.3e}") for k, v in state_errors(r["fields"], ref["fields"], ~dep).items()}
         for k in ("train_s", "train_samples", "train_loss"):
             if k in r:
                 s[k] = r[k]
@@ -253,6 +248,8 @@ def main():
     achieved = bpl * cells / launch_s / 1e9
     flops = FLOP_PER_CELL_ITER[case.lattice] * head["niter_mean"] * cells
     err = summary[head_name].get("rel_linf_vs_fp64")
+    kernel = KERNEL_NAME[(case.lattice, prec)]
+    traffic, traffic_src = profiled_traffic(kernel, f"{case.name}-{case.lattice}-{'x'.join(map(str, case.shape))}")
     out = {
         "metric": METRIC,
         "value": round(head["mlups"], 4),
@@ -275,8 +272,8 @@ def main():
         "speedup_vs_fp64": round(head["mlups"] / ref["mlups"], 3) if ref is not None and head is not ref else None,
         "paths": summary,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "bytes_per_lup": bpl,
-                     "kernel": KERNEL_NAME[(case.lattice, prec)]},
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "bytes_per_lup": bpl,
+                     "algorithmic_bytes_per_launch": bpl * cells, "kernel": kernel, "traffic_source": traffic_src},
         "valu_roofline": {"bound": "fp64-valu", "achieved": round(flops / launch_s / 1e12, 4), "peak": FP64_PEAK_TFLOPS,
                           "unit": "TFLOP/s", "frac": round(flops / launch_s / 1e12 / FP64_PEAK_TFLOPS, 5),
                           "flop_per_cell_iter": FLOP_PER_CELL_ITER[case.lattice],

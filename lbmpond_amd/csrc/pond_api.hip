@@ -284,6 +284,13 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
   // tuning switches: POND_XCD=0 launch-order workgroups, POND_NO_BUF=1 global loads (FP64)
   g.xcd = 1;
   if (const char* v = std::getenv("POND_XCD")) g.xcd = std::atoi(v) != 0;
+  // Row patches of the linear tile order (POND_ROW_PATCH=edge overrides; ignored where the grid
+  // does not divide into them).  Measured on the shock-bubble D3Q27 WENO5C at 384^3: split step
+  // 1124 ms in row order, 1025 ms with 4x4 patches, 1063-1075 ms with 8, 16 or 32; the FP64
+  // step 12.70 s in row order, 12.92 s with 8x8 patches.
+  int rp = p->precision == POND_SPLIT ? 4 : 0;
+  if (const char* v = std::getenv("POND_ROW_PATCH")) rp = std::atoi(v);
+  g.rp = (rp > 0 && g.tx == 64 && g.nx % 64 == 0 && g.ny % rp == 0 && g.nz % rp == 0) ? rp : 0;
   a.g = g;
   a.no_buf = 0;
   if (const char* v = std::getenv("POND_NO_BUF")) a.no_buf = std::atoi(v) != 0;
@@ -333,6 +340,8 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
     }
     SplitArgs& sa = ctx->sargs;
     sa.pass1_iters = a.pass1_iters;
+    sa.learned_trust = 0.1f;
+    if (const char* v = std::getenv("POND_LEARNED_TRUST")) sa.learned_trust = (float)std::atof(v);
     sa.weno5c_variant = 0;
     if (const char* v = std::getenv("POND_SPLIT_VARIANT")) sa.weno5c_variant = std::atoi(v);
     sa.g = g;

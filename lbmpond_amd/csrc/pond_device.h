@@ -18,6 +18,10 @@ struct Grid {
   int tx, ty, tz;
   int ntx, nty, ntz;
   int xcd;  // 1: XCD-contiguous workgroup order (xcd_block), 0: launch order
+  // linear tiles only: rows (y, z) taken in rp x rp patches (y fastest within a patch, patches
+  // y fastest then z) instead of y fastest over the whole plane; 0 = plain row order.  Needs
+  // nx % 64 == 0, ny % rp == 0 and nz % rp == 0.
+  int rp;
 };
 
 // Parameters of one step, passed by value to the kernels.
@@ -104,6 +108,18 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks) {
 // cell (x, y, z) of lane `lane` in tile `b` (tiles x fastest); false outside the grid.  The
 // 64x1x1 tile is the linear order: wave b holds cells 64b .. 64b+63, across row ends.
 __device__ __forceinline__ bool tile_cell(const Grid& g, int64_t b, int lane, int& x, int& y, int& z) {
+  if (g.tx == 64 && g.rp > 0) {
+    // the workgroups resident on one XCD at a time then cover a compact (y, z) patch of rows,
+    // whose stencil halo is smaller than that of a run of rows along y
+    const int wpr = g.nx / 64;
+    const int64_t row = b / wpr;
+    const int64_t pp = (int64_t)g.rp * g.rp, t = row / pp;
+    const int w = (int)(row % pp), npy = g.ny / g.rp;
+    x = (int)(b % wpr) * 64 + lane;
+    y = (int)(t % npy) * g.rp + w % g.rp;
+    z = (int)(t / npy) * g.rp + w / g.rp;
+    return row < (int64_t)g.ny * g.nz;
+  }
   if (g.tx == 64) {
     const int64_t c = b * 64 + lane;
     x = (int)(c % g.nx);

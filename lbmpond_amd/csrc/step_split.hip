@@ -391,10 +391,10 @@ __global__ void k_sqrt_t(const double* __restrict__ T, float* __restrict__ sq, i
 // previous gauge.  Cells that need this many are the ill-conditioned ones (slow contraction at
 // contact discontinuities), whose converged state can depend on the starting point.
 constexpr int kLearnedBudget = 12;
-// Largest predicted change (in thermal units: du/sqrt(T), dT/T) used as a start.  Larger changes
-// sit on shocks and contact layers, where the gauge map can have separate fixed points and a
-// start far from the previous gauge may settle on another one than the reference reaches.
-constexpr float kLearnedTrust = 0.1f;
+// SplitArgs::learned_trust: the largest predicted change (in thermal units: du/sqrt(T), dT/T)
+// used as a start.  Larger changes sit on shocks and contact layers, where the gauge map can have
+// separate fixed points and a start far from the previous gauge may settle on another one than
+// the reference reaches.
 
 // Iteration state of one cell, carried from the first pass to the second.
 struct SplitIter {
@@ -579,7 +579,7 @@ __device__ __forceinline__ void split27_pass1(const float* __restrict__ fin, con
       const float p0 = guess[cell], p1 = guess[P + cell], p2 = guess[2 * P + cell], p3 = guess[3 * P + cell];
       const double q[4] = {gp[0] + sq * p0, gp[1] + sq * p1, gp[2] + sq * p2, gp[3] * (1.0 + (double)p3)};
       const float pmax = fmaxf(fmaxf(fabsf(p0), fabsf(p1)), fmaxf(fabsf(p2), fabsf(p3)));
-      if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0 && pmax <= kLearnedTrust) {
+      if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0 && pmax <= a.learned_trust) {
         for (int k = 0; k < 4; k++) st.gc[k] = q[k];
         st.learned = true;
       }
@@ -720,7 +720,7 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
                          gp[3] * (1.0 + (double)guess[3 * P + cell])};
     const float pmax = fmaxf(fmaxf(fabsf(guess[cell]), fabsf(guess[P + cell])),
                              fmaxf(fabsf(guess[2 * P + cell]), fabsf(guess[3 * P + cell])));
-    if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0 && pmax <= kLearnedTrust) {
+    if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0 && pmax <= a.learned_trust) {
       for (int d = 0; d < 4; d++) gc[d] = q[d];
       learned = true;
     }
