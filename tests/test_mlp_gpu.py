@@ -41,11 +41,14 @@ def test_mfma_gradients_match_oracle(gpu):
 
 
 def test_learned_start_reduces_iterations(gpu):
-    """Train on the solver's own steps of a small shock-bubble case, then continue the same
-    state with and without the learned starting gauge: fewer gauge iterations, and the two runs
-    agree to 1e-5 outside the cells that an iteration-capped cell can have influenced (at such a
-    cell the loop ends on an iterate that depends on where it started; lbmpond_amd.compare)."""
-    case = ic.shock_bubble(n=32)
+    """Train on the solver's own steps of a small compressible-vortex case (Mach 1.2 uniform flow
+    with an isentropic vortex), then continue the same state with and without the learned
+    starting gauge: fewer gauge iterations, and the two runs agree to 1e-5 outside the cells
+    that an iteration-capped cell can have influenced (at such a cell the loop ends on an
+    iterate that depends on where it started; lbmpond_amd.compare).  On the small shock-bubble
+    grids the bubble's contact layer leaves cells at the cap in every step and their influence
+    covers nearly the whole grid within a few steps, so that case cannot show agreement."""
+    case = ic.vortex_3d(n=32)
     with Pond("D3Q27", case.shape, "weno5c", "split", use_mlp=True, dtau=case.dtau) as p:
         p.set_state(*case.fields)
         for s in range(12):
@@ -79,8 +82,8 @@ def test_learned_start_reduces_iterations(gpu):
     print(f"  capped cells per step: learned {dep_mlp.capped} plain {dep_plain.capped};"
           f" {dep.mean():.3f} of the cells excluded")
     print("  learned vs plain start, all cells:", {k: f"{v:.2e}" for k, v in state_errors(out_mlp, out_plain).items()})
-    assert dep.mean() < 0.5
-    errs = state_errors(out_mlp, out_plain, ~dep)
+    assert dep.mean() < 0.1
+    errs = state_errors(out_mlp, out_plain, ~dep if dep.any() else None)
     for k in STATE_ERRORS:
         print(f"  learned vs plain start {k}: {errs[k]:.2e}")
         assert errs[k] < 1e-5
