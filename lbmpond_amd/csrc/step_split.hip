@@ -415,18 +415,14 @@ __device__ __forceinline__ void split_prev_gauge(const float* __restrict__ min, 
   if (rho0 == 0 && gp[0] == 0 && gp[1] == 0 && gp[2] == 0 && gp[3] == 0) { rho0 = 1; gp[3] = tlat; }
 }
 
-// Fixed-point iterations of one cell from state st until it converges (true), its count
-// reaches stop_at, or its count since its current start reaches the cap (false).  A cell that
-// abandons a predicted start gets the full cap from its previous gauge, so a cell that does
-// not converge ends on the same iterate as without the prediction.  The iterate's populations
-// are left in fout.
+// One fixed-point iteration of one cell: true when it has converged.
 template <int INTERP>
-__device__ bool split27_iterate(const NodeSrcF& src, float* __restrict__ fout, int64_t cell, int x, int y, int z,
-                                const double gp[4], SplitIter& st, int stop_at, const SplitArgs& a) {
+__device__ __forceinline__ bool split27_once(const NodeSrcF& src, float* __restrict__ fout, int64_t cell, int x,
+                                             int y, int z, const double gp[4], SplitIter& st, const SplitArgs& a) {
   const int64_t P = a.g.plane;
   const float sqTb = (float)a.sqrt_tbase;
   double* gc = st.gc;
-  while (st.niter < stop_at && st.niter - st.base < a.max_iter) {
+  {
     st.niter++;
     const double s = sqrt(gc[3] / a.tlat);
     // moments in the frame of the current gauge: A = sum f, B = sum f xi, C = sum f |xi|^2
@@ -514,11 +510,24 @@ __device__ bool split27_iterate(const NodeSrcF& src, float* __restrict__ fout, i
       st.contracted = false;
       st.res_prev = 0;
       st.base = st.niter;
-      continue;
+      return false;
     }
     st.res_prev = res;
     for (int k = 0; k < 4; k++) gc[k] = m[k];
   }
+  return false;
+}
+
+// Fixed-point iterations of one cell from state st until it converges (true), its count
+// reaches stop_at, or its count since its current start reaches the cap (false).  A cell that
+// abandons a predicted start gets the full cap from its previous gauge, so a cell that does
+// not converge ends on the same iterate as without the prediction.  The iterate's populations
+// are left in fout.
+template <int INTERP>
+__device__ bool split27_iterate(const NodeSrcF& src, float* __restrict__ fout, int64_t cell, int x, int y, int z,
+                                const double gp[4], SplitIter& st, int stop_at, const SplitArgs& a) {
+  while (st.niter < stop_at && st.niter - st.base < a.max_iter)
+    if (split27_once<INTERP>(src, fout, cell, x, y, z, gp, st, a)) return true;
   return false;
 }
 
@@ -542,6 +551,34 @@ __device__ __forceinline__ void split27_finish(float* __restrict__ fout, float* 
   niter_out[cell] = st.niter;
 }
 
+// Starting state of a cell: its previous gauge, or the predicted one where a prediction is
+// given, finite, has T > 0 and lies within the trust radius.
+__device__ __forceinline__ void split27_start(const float* __restrict__ min, const double* __restrict__ Tin,
+                                              const float* __restrict__ guess, int64_t cell, double gp[4],
+                                              SplitIter& st, const SplitArgs& a) {
+  const int64_t P = a.g.plane;
+  double rho0;
+  split_prev_gauge(min, Tin, P, cell, a.tlat, gp, rho0);
+  for (int k = 0; k < 4; k++) st.gc[k] = gp[k];
+  st.res_prev = 0;
+  st.rho = rho0;
+  st.niter = 0;
+  st.base = 0;
+  st.learned = false;
+  st.contracted = false;
+  if (guess) {
+    // decode the predicted increment: (du/sqrt(T), dT/T) relative to the previous gauge
+    const double sq = sqrt(gp[3]);
+    const float p0 = guess[cell], p1 = guess[P + cell], p2 = guess[2 * P + cell], p3 = guess[3 * P + cell];
+    const double q[4] = {gp[0] + sq * p0, gp[1] + sq * p1, gp[2] + sq * p2, gp[3] * (1.0 + (double)p3)};
+    const float pmax = fmaxf(fmaxf(fabsf(p0), fabsf(p1)), fmaxf(fabsf(p2), fabsf(p3)));
+    if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0 && pmax <= a.learned_trust) {
+      for (int k = 0; k < 4; k++) st.gc[k] = q[k];
+      st.learned = true;
+    }
+  }
+}
+
 // First pass: every cell iterates up to a.pass1_iters times.  A wave runs as long as its
 // slowest cell, and on contact discontinuities and shocks a thin layer of cells needs several
 // times the mean, so cells still unconverged after the first pass are appended to a list
@@ -563,27 +600,8 @@ __device__ __forceinline__ void split27_pass1(const float* __restrict__ fin, con
   SplitIter st;
   const int64_t cell = live ? cell_index(g, x, y, z) : 0;
   if (live) {
-    const int64_t P = g.plane;
-    double gp[4], rho0;
-    split_prev_gauge(min, Tin, P, cell, a.tlat, gp, rho0);
-    for (int k = 0; k < 4; k++) st.gc[k] = gp[k];
-    st.res_prev = 0;
-    st.rho = rho0;
-    st.niter = 0;
-    st.base = 0;
-    st.learned = false;
-    st.contracted = false;
-    if (guess) {
-      // decode the predicted increment: (du/sqrt(T), dT/T) relative to the previous gauge
-      const double sq = sqrt(gp[3]);
-      const float p0 = guess[cell], p1 = guess[P + cell], p2 = guess[2 * P + cell], p3 = guess[3 * P + cell];
-      const double q[4] = {gp[0] + sq * p0, gp[1] + sq * p1, gp[2] + sq * p2, gp[3] * (1.0 + (double)p3)};
-      const float pmax = fmaxf(fmaxf(fabsf(p0), fabsf(p1)), fmaxf(fabsf(p2), fabsf(p3)));
-      if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0 && pmax <= a.learned_trust) {
-        for (int k = 0; k < 4; k++) st.gc[k] = q[k];
-        st.learned = true;
-      }
-    }
+    double gp[4];
+    split27_start(min, Tin, guess, cell, gp, st, a);
     const NodeSrcF src = make_src(fin, min, sqin, g);
     const bool conv = split27_iterate<INTERP>(src, fout, cell, x, y, z, gp, st, a.pass1_iters, a);
     pending = !conv && st.niter - st.base < a.max_iter;
@@ -616,6 +634,59 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const float* __restrict__ sqin, float* __restrict__ fout, float* __restrict__ mout, double* __restrict__ Tout,
     int* __restrict__ niter_out, const float* __restrict__ guess, SplitPending pend, const SplitArgs a) {
   split27_pass1<2>(fin, min, Tin, sqin, fout, mout, Tout, niter_out, guess, pend, a);
+}
+
+// Single pass with lane refill.  A wave owns a chunk of `chunk` consecutive 64-cell tiles.  Its
+// lanes start on the first tile; whenever lanes finish their cells, they take the chunk's next
+// unstarted cells in order (one ballot per iteration), so a lane is idle only at the end of the
+// chunk instead of for as long as the slowest cell of its tile iterates.  On the shock-bubble
+// the mean gauge iterations per cell are about 1.3 while the slowest cell of a 64-cell tile
+// needs 3.5-5, so lanes of the one-tile-per-wave step sit idle most of the time.  Each cell's
+// arithmetic is the same as in k_step_split27 (bitwise the same results); the active cells of
+// a wave stay within a window of the chunk, so node loads remain close together.
+template <int INTERP>
+__global__ __launch_bounds__(64) void k_step_split27_refill(
+    const float* __restrict__ fin, const float* __restrict__ min, const double* __restrict__ Tin,
+    const float* __restrict__ sqin, float* __restrict__ fout, float* __restrict__ mout, double* __restrict__ Tout,
+    int* __restrict__ niter_out, const float* __restrict__ guess, const SplitArgs a, int chunk) {
+  const Grid& g = a.g;
+  const int lane = threadIdx.x;
+  const int64_t ntiles = (int64_t)g.ntx * g.nty * g.ntz;
+  const int64_t w = xcd_block(blockIdx.x, g.xcd ? (int64_t)gridDim.x : -1);
+  const int64_t t0 = w * chunk;
+  const int64_t total = (ntiles - t0 < chunk ? ntiles - t0 : chunk) * 64;  // cell slots of the chunk
+  const NodeSrcF src = make_src(fin, min, sqin, g);
+  int64_t next = 0;  // first unstarted slot (the same in every lane)
+  bool active = false;
+  int64_t cell = 0;
+  int x = 0, y = 0, z = 0;
+  double gp[4] = {0, 0, 0, 0};
+  SplitIter st;
+  while (true) {
+    // lanes without a cell take the next slots, in lane order; a slot outside the grid (ragged
+    // last tile) is used up without a cell
+    const uint64_t want = __ballot(!active);
+    const int rank = __popcll(want & ((1ull << lane) - 1ull));
+    if (!active) {
+      const int64_t slot = next + rank;
+      if (slot < total && tile_cell(g, t0 + slot / 64, (int)(slot % 64), x, y, z)) {
+        cell = cell_index(g, x, y, z);
+        split27_start(min, Tin, guess, cell, gp, st, a);
+        active = true;
+      }
+    }
+    next += __popcll(want);
+    if (__ballot(active) == 0 && next >= total) break;
+    if (active) {
+      // the loop condition of split27_iterate, one iteration per trip
+      bool done = st.niter - st.base >= a.max_iter;
+      if (!done) done = split27_once<INTERP>(src, fout, cell, x, y, z, gp, st, a) || st.niter - st.base >= a.max_iter;
+      if (done) {
+        split27_finish(fout, mout, Tout, niter_out, cell, st, a);
+        active = false;
+      }
+    }
+  }
 }
 
 // Second pass: a fixed grid of waves walks the pending list; every cell resumes its iteration
@@ -888,6 +959,12 @@ hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const
     hipLaunchKernelGGL((k_step_split27_rest<I>), dim3(blocks2), dim3(64), 0, st, in.f, in.m, in.T, sq, out.f,     \
                        out.m, out.T, niter, pend, a);                                                             \
   } while (0)
+  if (interp == 2 && a.weno5c_variant == 0 && a.refill_chunk > 0 && a.pass1_iters >= a.max_iter + kLearnedBudget) {
+    const int64_t waves = ((int64_t)blocks + a.refill_chunk - 1) / a.refill_chunk;
+    hipLaunchKernelGGL(k_step_split27_refill<2>, dim3((unsigned)waves), dim3(64), 0, st, in.f, in.m, in.T, sq, out.f,
+                       out.m, out.T, niter, guess, a, a.refill_chunk);
+    return hipGetLastError();
+  }
   if (interp == 1) POND_LAUNCH_S(k_step_split27<1>, 1);
   else if (interp == 2 && a.weno5c_variant == 1) POND_LAUNCH_S(k_step_split27_g3, 2);
   else if (interp == 2 && a.weno5c_variant == 2) POND_LAUNCH_S(k_step_split27<3>, 3);

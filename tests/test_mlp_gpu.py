@@ -40,29 +40,25 @@ def test_mfma_gradients_match_oracle(gpu):
         assert e < 1e-2, name
 
 
-def test_learned_start_reduces_iterations(gpu):
-    """Train on the solver's own steps of a small compressible-vortex case (Mach 1.2 uniform flow
-    with an isentropic vortex), then continue the same state with and without the learned
-    starting gauge: fewer gauge iterations, and the two runs agree to 1e-5 outside the cells
-    that an iteration-capped cell can have influenced (at such a cell the loop ends on an
-    iterate that depends on where it started; lbmpond_amd.compare).  On the small shock-bubble
-    grids the bubble's contact layer leaves cells at the cap in every step and their influence
-    covers nearly the whole grid within a few steps, so that case cannot show agreement."""
-    case = ic.vortex_3d(n=32)
+def learned_vs_plain(case, collect=12, cont=6):
+    """Train on `collect` steps of the solver's own cells, then continue the same state for `cont`
+    steps with and without the learned start.  Returns the training losses, the fraction of cells
+    whose first predicted start lies within the trust radius, both runs' mean iterations,
+    final fields and cap-dependence trackers."""
     with Pond("D3Q27", case.shape, "weno5c", "split", use_mlp=True, dtau=case.dtau) as p:
         p.set_state(*case.fields)
-        for s in range(12):
+        for s in range(collect):
             p.step(1)
             p.mlp_collect(16384, seed=100 + s)
         losses = p.mlp_train(epochs=8, batch=1024, lr=2e-3, seed=7)
         print("training losses per epoch:", losses)
-        assert losses[-1] < losses[0]
         f = p.populations()
         st = p.fields()
-        params, scales = p.mlp_params()
+        pred = p.mlp_predict()
+        used = float((np.abs(pred).max(axis=0) <= 0.1).mean())
         iters_mlp = []
         dep_mlp = CapDependence(case.shape, p.params.max_iter)
-        for _ in range(6):
+        for _ in range(cont):
             p.step(1)
             iters_mlp.append(p.diagnostics()["niter_mean"])
             dep_mlp.update(p.niter())
@@ -71,19 +67,40 @@ def test_learned_start_reduces_iterations(gpu):
         q.set_populations(f.reshape(-1), st["rho"], st["ux"], st["uy"], st["uz"], st["T"])
         iters_plain = []
         dep_plain = CapDependence(case.shape, q.params.max_iter)
-        for _ in range(6):
+        for _ in range(cont):
             q.step(1)
             iters_plain.append(q.diagnostics()["niter_mean"])
             dep_plain.update(q.niter())
         out_plain = q.fields()
-    print("mean gauge iterations: learned", np.mean(iters_mlp), "plain", np.mean(iters_plain))
-    assert np.mean(iters_mlp) < np.mean(iters_plain)
     dep = dep_mlp.mask | dep_plain.mask
+    print("  mean gauge iterations: learned", np.mean(iters_mlp), "plain", np.mean(iters_plain),
+          f"; predicted start within the trust radius in {used:.3f} of the cells")
     print(f"  capped cells per step: learned {dep_mlp.capped} plain {dep_plain.capped};"
-          f" {dep.mean():.3f} of the cells excluded")
+          f" {dep.mean():.3f} of the cells in their region of influence")
     print("  learned vs plain start, all cells:", {k: f"{v:.2e}" for k, v in state_errors(out_mlp, out_plain).items()})
-    assert dep.mean() < 0.1
-    errs = state_errors(out_mlp, out_plain, ~dep if dep.any() else None)
+    return losses, used, np.mean(iters_mlp), np.mean(iters_plain), out_mlp, out_plain, dep
+
+
+def test_learned_start_reduces_iterations(gpu):
+    """Shock-bubble 32^3: the network trained on 12 steps of the solver's own cells lowers the mean
+    gauge iterations of the next 6 steps against the plain start from the same state.  (On this
+    grid the bubble's contact layer leaves cells at the cap in every step, and their region of
+    influence covers nearly the whole grid within a few steps; agreement of the two runs is
+    tested on a cap-free case below.)"""
+    losses, used, it_mlp, it_plain, _, _, _ = learned_vs_plain(ic.shock_bubble(n=32))
+    assert losses[-1] < losses[0]
+    assert used > 0.5
+    assert it_mlp < it_plain
+
+
+def test_learned_start_agrees_with_plain_start(gpu):
+    """Compressible vortex 32^3 (Mach 1.2 flow, no cell reaches the cap): continuing the same state
+    with and without the learned start gives the same fields to 1e-5 in every cell, with the
+    prediction taken as the start in most cells."""
+    _, used, _, _, out_mlp, out_plain, dep = learned_vs_plain(ic.vortex_3d(n=32))
+    assert used > 0.5
+    assert not dep.any()
+    errs = state_errors(out_mlp, out_plain)
     for k in STATE_ERRORS:
         print(f"  learned vs plain start {k}: {errs[k]:.2e}")
         assert errs[k] < 1e-5

@@ -62,8 +62,10 @@ def test_split_lagr4_random(gpu):
 
 
 def test_split_taylor_green_small(gpu):
+    """20 steps: in step 18 a few cells reach the iteration cap in both runs; the cells they can
+    have influenced by step 20 (about a fifth of this small grid) are left out."""
     case = ic.taylor_green(n=32, steps=20)
-    compare(case.shape, "weno5c", case.fields, 20, 1e-5, dtau=case.dtau)
+    compare(case.shape, "weno5c", case.fields, 20, 1e-5, dtau=case.dtau, max_dependent=0.3)
 
 
 def test_split_taylor_green_error_tracks_fp64_sensitivity(gpu):
@@ -147,3 +149,23 @@ def test_two_pass_split_is_independent_of_first_pass_length(gpu, monkeypatch):
     for o in outs[:2]:
         for key in ("rho", "ux", "uy", "uz", "T", "niter"):
             assert np.array_equal(o[key], outs[2][key]), key
+
+
+def test_refill_matches_one_tile_per_wave(gpu, monkeypatch):
+    """The lane-refill single pass (a wave owns several 64-cell tiles and its lanes take new cells
+    as theirs converge) gives bitwise the results of one tile per wave, on a shock-bubble state
+    and on a grid whose last tile is ragged."""
+    cases = ((ic.shock_bubble(n=40).fields, (40, 40, 40), 4),
+             (smooth_random_state((37, 20, 11), seed=71, T0=0.01), (37, 20, 11), 3))
+    for fields, shape, steps in cases:
+        outs = []
+        for chunk in ("0", "1", "3", "8"):
+            monkeypatch.setenv("POND_REFILL_CHUNK", chunk)
+            print(f"[refill] {shape} chunk {chunk}", flush=True)
+            with Pond("D3Q27", shape, "weno5c", "split") as p:
+                p.set_state(*fields)
+                p.step(steps)
+                outs.append(p.fields())
+        for o in outs[1:]:
+            for k in ("rho", "ux", "uy", "uz", "T", "niter"):
+                assert np.array_equal(o[k], outs[0][k]), (shape, k)
