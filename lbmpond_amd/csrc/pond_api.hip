@@ -341,10 +341,11 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
     SplitArgs& sa = ctx->sargs;
     sa.pass1_iters = a.pass1_iters;
     // lane refill over chunks of 64-cell tiles (POND_REFILL_CHUNK overrides, 0 = one tile per
-    // wave).  Measured on the 384^3 shock-bubble, split step ms: one tile 1016, chunks of 2 tiles
-    // 940, 4 984, 8 1252, 16 1559, 32 1735: the idle lanes of a tile cost less than the loss of
-    // coalescing once the active cells of a wave spread over more than two tiles.
-    sa.refill_chunk = 2;
+    // wave).  Measured on the 384^3 shock-bubble, split step ms, with paired x-slice loads: one
+    // tile 1065, chunks of 2 tiles 805, 3 976, 4 775-778, 6 763, 8 840, 12 910 (one slice per
+    // load and one tile per wave: 1025).  Larger chunks keep more lanes busy but spread the
+    // active cells of a wave over more rows, so its loads touch more cache lines.
+    sa.refill_chunk = 4;
     if (const char* v = std::getenv("POND_REFILL_CHUNK")) sa.refill_chunk = std::atoi(v) > 0 ? std::atoi(v) : 0;
     sa.learned_trust = 0.1f;
     if (const char* v = std::getenv("POND_LEARNED_TRUST")) sa.learned_trust = (float)std::atof(v);

@@ -107,6 +107,14 @@ __device__ __forceinline__ float ldf(const PlanesF& p, int k, uint32_t boff) {
                                                                         (int)((uint32_t)(k % kPlanesPerRsrc) * p.pbytes), 0));
 }
 
+// plane k of two neighbouring cells at byte offsets boff and boff + 4 in one 8-byte load
+__device__ __forceinline__ void ldf2(const PlanesF& p, int k, uint32_t boff, float& a, float& b) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(p.r[k / kPlanesPerRsrc], (int)boff,
+                                                      (int)((uint32_t)(k % kPlanesPerRsrc) * p.pbytes), 0);
+  a = __builtin_bit_cast(float, (uint32_t)v[0]);
+  b = __builtin_bit_cast(float, (uint32_t)v[1]);
+}
+
 // sum-factorised gauge transfer of the node at byte offset boff to target t.  The gauge
 // difference u1 - u0 is formed as (hi - u0) + lo: hi - u0 is exact when the two velocities are
 // within a factor of two (Sterbenz), so the difference carries one rounding relative to itself
@@ -267,6 +275,7 @@ __device__ float interp_lagr4f(const NodeSrcF& src, double cx, double cy, double
 // five-slice WENO windows are visited once each (their union, at most 5 + |r(-1) - r(+1)|).
 // Same interpolation as interp_weno5f<true> per target, with the y/z weights summed in a
 // different order.
+template <bool PAIRS>
 __device__ void interp_group_weno5c(const NodeSrcF& src, int x, int y, int z, double s, const double gc[4], int ey,
                                     int ez, float sqTb, float out[3]) {
   const Grid& g = src.g;
@@ -305,50 +314,39 @@ __device__ void interp_group_weno5c(const NodeSrcF& src, int x, int y, int z, do
 #pragma unroll
     for (int j = 0; j < 5; j++) ub[t][j] = 0.f;
   }
-  for (int xs = xlo; xs <= xhi; xs++) {
-    const int xw = wrapi(xs, g.nx);
-    float yz[3] = {0.f, 0.f, 0.f}, mid[3] = {0.f, 0.f, 0.f};
-    // one node at a time: its 31 loads are the only ones in flight per lane
-#pragma unroll 1
-    for (int q = 0; q < 9; q++) {
-      const int a = q / 3, b = q - 3 * (q / 3);
-      const int ya = a == 0 ? yy[0] : a == 1 ? yy[1] : yy[2];
-      const int zb = b == 0 ? zz[0] : b == 1 ? zz[1] : zz[2];
-      const float fya = a == 0 ? Fy[0] : a == 1 ? Fy[1] : Fy[2];
-      const float fzb = b == 0 ? Fz[0] : b == 1 ? Fz[1] : Fz[2];
-      {
-        const uint32_t boff = (uint32_t)cell_index(g, xw, ya, zb) * 4u;
-        const float u0x = ldf(src.m, 1, boff), u0y = ldf(src.m, 2, boff), u0z = ldf(src.m, 3, boff);
-        const float sq0 = ldf(src.sq, 0, boff);
-        float Ly[3], Lz[3];
-        basis3(((u1y - u0y) + u1yl) * sqTb, sq0, ky, cyb, T1, wy, Ly);
-        basis3(((u1z - u0z) + u1zl) * sqTb, sq0, kz, czb, T1, wz, Lz);
-        float sy[3];
+  // contribution of one stencil node (its moments and 27 populations) to the three targets'
+  // y/z sums yz (weight wab) and, for the middle node, to their smoothness samples mid
+  auto node = [&](float u0x, float u0y, float u0z, float sq0, const float f[27], float wab, bool is_mid, float yz[3],
+                  float mid[3]) {
+    float Ly[3], Lz[3];
+    basis3(((u1y - u0y) + u1yl) * sqTb, sq0, ky, cyb, T1, wy, Ly);
+    basis3(((u1z - u0z) + u1zl) * sqTb, sq0, kz, czb, T1, wz, Lz);
+    float sy[3];
 #pragma unroll
-        for (int ja = 0; ja < 3; ja++) {
-          float acc = 0.f;
+    for (int ja = 0; ja < 3; ja++) {
+      float acc = 0.f;
 #pragma unroll
-          for (int jb = 0; jb < 3; jb++) {
-            float sz = 0.f;
+      for (int jb = 0; jb < 3; jb++) {
+        float sz = 0.f;
 #pragma unroll
-            for (int jc = 0; jc < 3; jc++) sz = __builtin_fmaf(Lz[jc], ldf(src.f, kIdx27.v[ja][jb][jc], boff), sz);
-            acc = __builtin_fmaf(Ly[jb], sz, acc);
-          }
-          sy[ja] = acc;
-        }
-        const float Dx = ((u1x - u0x) + u1xl) * sqTb;
-        const float wab = fya * fzb;
-#pragma unroll
-        for (int t = 0; t < 3; t++) {
-          float Lx[3];
-          basis3(Dx, sq0, t != 1, (float)(t - 1) * sq1, T1, t != 1 ? h : mm, Lx);
-          const float T = __builtin_fmaf(Lx[2], sy[2], __builtin_fmaf(Lx[1], sy[1], Lx[0] * sy[0]));
-          yz[t] = __builtin_fmaf(wab, T, yz[t]);
-          if (q == 4) mid[t] = T;
-        }
+        for (int jc = 0; jc < 3; jc++) sz = __builtin_fmaf(Lz[jc], f[kIdx27.v[ja][jb][jc]], sz);
+        acc = __builtin_fmaf(Ly[jb], sz, acc);
       }
+      sy[ja] = acc;
     }
-    // slice xs enters target t's window at position j = xs - (r_t - 2)
+    const float Dx = ((u1x - u0x) + u1xl) * sqTb;
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+      float Lx[3];
+      basis3(Dx, sq0, t != 1, (float)(t - 1) * sq1, T1, t != 1 ? h : mm, Lx);
+      const float T = __builtin_fmaf(Lx[2], sy[2], __builtin_fmaf(Lx[1], sy[1], Lx[0] * sy[0]));
+      yz[t] = __builtin_fmaf(wab, T, yz[t]);
+      if (is_mid) mid[t] = T;
+    }
+  };
+  // slice xs (its y/z sums yz and middle values mid) enters target t's window at position
+  // j = xs - (r_t - 2)
+  auto slice = [&](int xs, const float yz[3], const float mid[3]) {
 #pragma unroll
     for (int t = 0; t < 3; t++) {
       const int j = xs - (r[t] - 2);
@@ -367,6 +365,60 @@ __device__ void interp_group_weno5c(const NodeSrcF& src, int x, int y, int z, do
         }
       }
     }
+  };
+  for (int xs = xlo; xs <= xhi; xs += PAIRS ? 2 : 1) {
+    const int xw = wrapi(xs, g.nx);
+    float yz[3] = {0.f, 0.f, 0.f}, mid[3] = {0.f, 0.f, 0.f};
+    float yz2[3] = {0.f, 0.f, 0.f}, mid2[3] = {0.f, 0.f, 0.f};
+    // PAIRS: slices xs and xs+1 together, each node and plane of the two by one 8-byte load
+    // where xs+1 is the next cell of the row in every lane of the wave (away from the periodic
+    // wrap); a lane whose window ends at xs loads the pair and ignores its second half
+    const bool two = PAIRS && xs + 1 <= xhi;
+    const int xw2 = PAIRS ? wrapi(xs + 1, g.nx) : 0;
+    const bool paired = PAIRS && __all(!two || xw2 == xw + 1);
+    // one node at a time: its loads are the only ones in flight per lane
+#pragma unroll 1
+    for (int q = 0; q < 9; q++) {
+      const int a = q / 3, b = q - 3 * (q / 3);
+      const int ya = a == 0 ? yy[0] : a == 1 ? yy[1] : yy[2];
+      const int zb = b == 0 ? zz[0] : b == 1 ? zz[1] : zz[2];
+      const float fya = a == 0 ? Fy[0] : a == 1 ? Fy[1] : Fy[2];
+      const float fzb = b == 0 ? Fz[0] : b == 1 ? Fz[1] : Fz[2];
+      const float wab = fya * fzb;
+      const uint32_t boff = (uint32_t)cell_index(g, xw, ya, zb) * 4u;
+      if (!PAIRS) {
+        float f[27];
+#pragma unroll
+        for (int k = 0; k < 27; k++) f[k] = ldf(src.f, k, boff);
+        node(ldf(src.m, 1, boff), ldf(src.m, 2, boff), ldf(src.m, 3, boff), ldf(src.sq, 0, boff), f, wab, q == 4, yz,
+             mid);
+      } else if (paired) {
+        float f[27], f2[27], mx[2], my[2], mz[2], sq[2];
+#pragma unroll
+        for (int k = 0; k < 27; k++) ldf2(src.f, k, boff, f[k], f2[k]);
+        ldf2(src.m, 1, boff, mx[0], mx[1]);
+        ldf2(src.m, 2, boff, my[0], my[1]);
+        ldf2(src.m, 3, boff, mz[0], mz[1]);
+        ldf2(src.sq, 0, boff, sq[0], sq[1]);
+        node(mx[0], my[0], mz[0], sq[0], f, wab, q == 4, yz, mid);
+        if (two) node(mx[1], my[1], mz[1], sq[1], f2, wab, q == 4, yz2, mid2);
+      } else {
+        float f[27];
+#pragma unroll
+        for (int k = 0; k < 27; k++) f[k] = ldf(src.f, k, boff);
+        node(ldf(src.m, 1, boff), ldf(src.m, 2, boff), ldf(src.m, 3, boff), ldf(src.sq, 0, boff), f, wab, q == 4, yz,
+             mid);
+        if (two) {
+          const uint32_t boff2 = (uint32_t)cell_index(g, xw2, ya, zb) * 4u;
+#pragma unroll
+          for (int k = 0; k < 27; k++) f[k] = ldf(src.f, k, boff2);
+          node(ldf(src.m, 1, boff2), ldf(src.m, 2, boff2), ldf(src.m, 3, boff2), ldf(src.sq, 0, boff2), f, wab, q == 4,
+               yz2, mid2);
+        }
+      }
+    }
+    slice(xs, yz, mid);
+    if (two) slice(xs + 1, yz2, mid2);
   }
 #pragma unroll
   for (int t = 0; t < 3; t++) out[t] = weno5_combine(c[t], ub[t], S[t]);
@@ -436,12 +488,12 @@ __device__ __forceinline__ bool split27_once(const NodeSrcF& src, float* __restr
       kahan_add(bzs, bzc, (float)ez * fn);
       cs += (double)fn * (double)(ex * ex + ey * ey + ez * ez);
     };
-    if constexpr (INTERP == 2) {
+    if constexpr (INTERP == 2 || INTERP == 4) {
       // WENO5C: the three targets of each (ey, ez) group share their node loads and y/z work
       for (int gi = 0; gi < 9; gi++) {
         const int ey = gi / 3 - 1, ez = gi % 3 - 1;
         float f3[3];
-        interp_group_weno5c(src, x, y, z, s, gc, ey, ez, sqTb, f3);
+        interp_group_weno5c<INTERP == 2>(src, x, y, z, s, gc, ey, ez, sqTb, f3);
         add(-1, ey, ez, f3[0]);
         add(0, ey, ez, f3[1]);
         add(1, ey, ez, f3[2]);
@@ -585,7 +637,8 @@ __device__ __forceinline__ void split27_start(const float* __restrict__ min, con
 // (wave-aggregated: one atomic per wave) and finished by k_step_split27_rest, whose waves
 // hold only such cells.  Each cell's arithmetic is the same as in a single pass.
 // INTERP: 0 WENO5 as written, 1 Lagrange-4, 2 WENO5C with the targets of an (ey, ez) group
-// sharing their node work, 3 WENO5C target by target.
+// sharing their node work and two x slices loaded together, 3 WENO5C target by target, 4 as 2
+// with one x slice at a time.
 template <int INTERP>
 __device__ __forceinline__ void split27_pass1(const float* __restrict__ fin, const float* __restrict__ min,
                                               const double* __restrict__ Tin, const float* __restrict__ sqin,
@@ -968,6 +1021,7 @@ hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const
   if (interp == 1) POND_LAUNCH_S(k_step_split27<1>, 1);
   else if (interp == 2 && a.weno5c_variant == 1) POND_LAUNCH_S(k_step_split27_g3, 2);
   else if (interp == 2 && a.weno5c_variant == 2) POND_LAUNCH_S(k_step_split27<3>, 3);
+  else if (interp == 2 && a.weno5c_variant == 3) POND_LAUNCH_S(k_step_split27<4>, 4);
   else if (interp == 2) POND_LAUNCH_S(k_step_split27<2>, 2);
   else POND_LAUNCH_S(k_step_split27<0>, 0);
 #undef POND_LAUNCH_S

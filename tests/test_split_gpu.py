@@ -151,17 +151,20 @@ def test_two_pass_split_is_independent_of_first_pass_length(gpu, monkeypatch):
             assert np.array_equal(o[key], outs[2][key]), key
 
 
-def test_refill_matches_one_tile_per_wave(gpu, monkeypatch):
+def test_refill_and_paired_loads_match_plain_kernel(gpu, monkeypatch):
     """The lane-refill single pass (a wave owns several 64-cell tiles and its lanes take new cells
-    as theirs converge) gives bitwise the results of one tile per wave, on a shock-bubble state
-    and on a grid whose last tile is ragged."""
+    as theirs converge) and the paired x-slice loads (two neighbouring cells' planes in one 8-byte
+    load) give bitwise the results of one tile per wave with one slice per load, on a
+    shock-bubble state and on a grid whose rows are shorter than a wave and whose last tile is
+    ragged (row ends inside a wave, where the pairs fall back to single loads)."""
     cases = ((ic.shock_bubble(n=40).fields, (40, 40, 40), 4),
              (smooth_random_state((37, 20, 11), seed=71, T0=0.01), (37, 20, 11), 3))
     for fields, shape, steps in cases:
         outs = []
-        for chunk in ("0", "1", "3", "8"):
+        for variant, chunk in (("3", "0"), ("0", "0"), ("0", "1"), ("0", "3"), ("0", "8")):
+            monkeypatch.setenv("POND_SPLIT_VARIANT", variant)
             monkeypatch.setenv("POND_REFILL_CHUNK", chunk)
-            print(f"[refill] {shape} chunk {chunk}", flush=True)
+            print(f"[refill] {shape} variant {variant} chunk {chunk}", flush=True)
             with Pond("D3Q27", shape, "weno5c", "split") as p:
                 p.set_state(*fields)
                 p.step(steps)
