@@ -56,8 +56,8 @@ __device__ __forceinline__ float raw_feature(const MlpState& s, int x, int y, in
 
 // ------------------------------------------------------------------------ fused inference
 
-// cells of the neighbourhood of a 16-cell row tile: 18 x 3 x 3
-constexpr int kStage = 18 * 9;
+// cells of the neighbourhood of a 64-cell row segment: 66 x 3 x 3
+constexpr int kStage = 66 * 9;
 
 // orders this wave's LDS writes before its following reads (waves stage independently)
 __device__ __forceinline__ void wave_sync() {
@@ -66,10 +66,25 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One wave per 16-cell tile loop.  Features are taken from the state (mode 0) or from a
+// Waves loop over 16-cell tiles.  Features are taken from the state (mode 0) or from a
 // normalized feature matrix (mode 1, for checking the network arithmetic).
+// Input feature held in slot k = 16*st + 4*q + e of the first layer's operands (st: K step,
+// q: lane group, e: element), or -1 for the 9 unused slots.  The 135 features (neighbour nb,
+// type: rho, ux, uy, uz, T) are laid out so that the type depends on (st, e) only: every lane
+// of a wave builds the same kind of feature at each unrolled (st, e), for different
+// neighbours, with no divergent branch.  The order of the sum over features inside the MFMA is
+// all this changes.
+__device__ __forceinline__ int slot_feature(int st, int q, int e) {
+  const int j = st * 4 + e;  // 0..35; seven (st, e) pairs per type
+  const int type = j / 7, nb = (j % 7) * 4 + q;
+  return (j < 35 && nb < 27) ? nb * 5 + type : -1;
+}
+
+// Two waves per SIMD (256 VGPRs, a few spilled): the weights held in registers take most of
+// them, and with one wave per SIMD nothing hides the LDS and MFMA latencies.  Measured on the
+// 384^3 shock-bubble: 14.9 ms per step at one wave per SIMD, 8.4 ms at two.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_mlp_infer(const MlpWeightsBF w, MlpState s, const float* __restrict__ Xin,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_mlp_infer(const MlpWeightsBF w, MlpState s, const float* __restrict__ Xin,
                                                   int64_t ncell, float* __restrict__ out, int64_t out_plane) {
   const int l = threadIdx.x & 63;
   const int q = l >> 4, col = l & 15;
@@ -81,7 +96,10 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const MlpWeightsBF w, MlpStat
     for (int st = 0; st < 9; st++) {
       short4_t v;
 #pragma unroll
-      for (int e = 0; e < 4; e++) v[e] = (short)w.W1[(16 * st + 4 * q + e) * kMlpH + 16 * t + col];
+      for (int e = 0; e < 4; e++) {
+        const int f = slot_feature(st, q, e);
+        v[e] = f >= 0 ? (short)w.W1[f * kMlpH + 16 * t + col] : (short)0;
+      }
       a1[t][st] = v;
     }
 #pragma unroll
@@ -113,77 +131,80 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const MlpWeightsBF w, MlpStat
 
   const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int64_t nwave = (int64_t)gridDim.x * (blockDim.x >> 6);
-  // MODE 0: tiles are 16 consecutive cells of one x-row (a row has ceil(nx/16) tiles), so the
-  // 3x3x3 neighbourhoods of a tile's cells lie in 18 x 3 x 3 cells; the wave stages their
-  // (rho, u, T) in LDS once and builds the 135 features of every cell from there.
-  __shared__ float lm[4][kStage][4];
-  __shared__ double lT[4][kStage];
+  // MODE 0: a wave takes 64 consecutive cells of one x-row at a time (a row has ceil(nx/64)
+  // such segments); the 3x3x3 neighbourhoods of its cells lie in 66 x 3 x 3 cells, whose
+  // (rho, u, T) it stages in LDS once, then it runs the network on four 16-cell tiles built
+  // from there.  Features are formed in FP32 (they enter the network as bf16).
+  __shared__ float lm[4][kStage][5];
   const int wv = threadIdx.x >> 6;
   const Grid& g = s.g;
-  const int ntr = (g.nx + 15) / 16;
-  const int64_t ntile = MODE == 0 ? (int64_t)ntr * g.ny * g.nz : (ncell + 15) / 16;
-  for (int64_t tile = wave; tile < ntile; tile += nwave) {
+  const int nseg = MODE == 0 ? (g.nx + 63) / 64 : 0;
+  const int64_t nunit = MODE == 0 ? (int64_t)nseg * g.ny * g.nz : (ncell + 15) / 16;
+  for (int64_t unit = wave; unit < nunit; unit += nwave) {
+    int xs0 = 0, y = 0, z = 0;
+    if (MODE == 0) {
+      xs0 = 64 * (int)(unit % nseg);
+      const int64_t row = unit / nseg;
+      y = (int)(row % g.ny);
+      z = (int)(row / g.ny);
+      wave_sync();  // the previous segment's reads of this wave's LDS are done
+      for (int e = l; e < kStage; e += 64) {
+        const int xx = e % 66, r = e / 66;
+        const int64_t c =
+            cell_index(g, wrapi(xs0 - 1 + xx, g.nx), wrapi(y + r / 3 - 1, g.ny), wrapi(z + r % 3 - 1, g.nz));
+#pragma unroll
+        for (int k = 0; k < 4; k++) lm[wv][e][k] = s.m[(int64_t)k * g.plane + c];
+        lm[wv][e][4] = (float)s.T[c];
+      }
+      wave_sync();
+    }
+#pragma unroll 1
+  for (int sub = 0; sub < (MODE == 0 ? 4 : 1); sub++) {
     int64_t cell;
     bool live;
     // B operands of layer 1: features 16*st + 4*q + e of this lane's cell
     short4_t xb[9];
     if (MODE == 0) {
-      const int xt = (int)(tile % ntr);
-      const int64_t row = tile / ntr;
-      const int y = (int)(row % g.ny), z = (int)(row / g.ny);
-      const int x = 16 * xt + col;
+      const int x = xs0 + 16 * sub + col;
       live = x < g.nx;
       cell = cell_index(g, live ? x : 0, y, z);
-      wave_sync();  // the previous tile's reads of this wave's LDS are done
-      for (int e = l; e < kStage; e += 64) {
-        const int xx = e % 18, r = e / 18;
-        const int64_t c =
-            cell_index(g, wrapi(16 * xt - 1 + xx, g.nx), wrapi(y + r / 3 - 1, g.ny), wrapi(z + r % 3 - 1, g.nz));
-#pragma unroll
-        for (int k = 0; k < 4; k++) lm[wv][e][k] = s.m[(int64_t)k * g.plane + c];
-        lT[wv][e] = s.T[c];
-      }
-      wave_sync();
-      const int ec = 4 * 18 + col + 1;  // the cell itself: dx = dy = dz = 0
-      const float rc = lm[wv][ec][0], uxc = lm[wv][ec][1], uyc = lm[wv][ec][2], uzc = lm[wv][ec][3];
-      const double Tc = lT[wv][ec];
-      const float isT = (float)(1.0 / sqrt(Tc));
+      const int ec = 4 * 66 + 16 * sub + col + 1;  // the cell itself: dx = dy = dz = 0
+      const float rci = 1.0f / lm[wv][ec][0], uxc = lm[wv][ec][1], uyc = lm[wv][ec][2], uzc = lm[wv][ec][3];
+      const float Tci = 1.0f / lm[wv][ec][4];
+      const float isT = 1.0f / sqrtf(lm[wv][ec][4]);
 #pragma unroll
       for (int st = 0; st < 9; st++) {
         short4_t v;
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-          const int f = 16 * st + 4 * q + e;
+          const int j = st * 4 + e;
+          const int type = j / 7, nb = (j % 7) * 4 + q;  // slot_feature: type fixed per (st, e)
           float val = 0.0f;
-          if (live && f < kMlpIn) {
+          if (j < 35 && live && nb < 27) {
             // raw_feature, read from the staged neighbourhood
-            const int nb = f / 5, type = f - 5 * (f / 5);
             const int dx = nb / 9 - 1, dyz = nb - 9 * (nb / 9);  // (dy+1)*3 + (dz+1)
-            const int en = dyz * 18 + col + 1 + dx;
-            float raw;
-            switch (type) {
-              case 0: raw = lm[wv][en][0] / rc - 1.0f; break;
-              case 1: raw = (lm[wv][en][1] - uxc) * isT; break;
-              case 2: raw = (lm[wv][en][2] - uyc) * isT; break;
-              case 3: raw = (lm[wv][en][3] - uzc) * isT; break;
-              default: raw = (float)(lT[wv][en] / Tc - 1.0); break;
-            }
-            val = raw * w.in_inv[type];
+            const float* c = lm[wv][dyz * 66 + 16 * sub + col + 1 + dx];
+            const float raw = type == 0   ? c[0] * rci - 1.0f
+                              : type == 1 ? (c[1] - uxc) * isT
+                              : type == 2 ? (c[2] - uyc) * isT
+                              : type == 3 ? (c[3] - uzc) * isT
+                                          : c[4] * Tci - 1.0f;
+            val = raw * w.in_inv[type < 5 ? type : 0];
           }
           v[e] = (short)f2bf(val);
         }
         xb[st] = v;
       }
     } else {
-      cell = tile * 16 + col;
+      cell = unit * 16 + col;
       live = cell < ncell;
 #pragma unroll
       for (int st = 0; st < 9; st++) {
         short4_t v;
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-          const int f = 16 * st + 4 * q + e;
-          v[e] = (short)f2bf(live && f < kMlpIn ? Xin[cell * kMlpIn + f] : 0.0f);
+          const int f = slot_feature(st, q, e);
+          v[e] = (short)f2bf(live && f >= 0 ? Xin[cell * kMlpIn + f] : 0.0f);
         }
         xb[st] = v;
       }
@@ -220,6 +241,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const MlpWeightsBF w, MlpStat
 #pragma unroll
       for (int e = 0; e < kMlpOut; e++) out[e * out_plane + cell] = (acc[e] + b3[e]) * w.out_scale[e];
     }
+  }
   }
 }
 
@@ -464,8 +486,8 @@ __global__ void k_moments2(const float* __restrict__ X, const float* __restrict_
 // ------------------------------------------------------------------------ launchers
 
 hipError_t mlp_launch_infer(const MlpWeightsBF& w, const MlpState& s, float* out, int64_t out_plane, hipStream_t st) {
-  const int64_t ntile = (int64_t)((s.g.nx + 15) / 16) * s.g.ny * s.g.nz;
-  const int64_t waves = ntile < 16384 ? ntile : 16384;
+  const int64_t nseg = (int64_t)((s.g.nx + 63) / 64) * s.g.ny * s.g.nz;
+  const int64_t waves = nseg < 16384 ? nseg : 16384;
   const unsigned blocks = (unsigned)((waves + 3) / 4);
   hipLaunchKernelGGL(k_mlp_infer<0>, dim3(blocks), dim3(256), 0, st, w, s, (const float*)nullptr, s.g.n, out, out_plane);
   return hipGetLastError();

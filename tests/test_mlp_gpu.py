@@ -129,12 +129,12 @@ def test_uniform_neighbourhood_keeps_plain_start(gpu):
         assert np.array_equal(outs[0][k], outs[1][k]), k
 
 
-def test_fused_inference_on_state_matches_oracle(gpu):
+@pytest.mark.parametrize("shape", [(20, 7, 5), (70, 5, 4), (128, 3, 3)])
+def test_fused_inference_on_state_matches_oracle(gpu, shape):
     """The step's fused inference (features gathered from the staged 3x3x3 neighbourhoods of
-    16-cell row tiles) against the oracle network on features built on the host, on a grid
-    whose rows are not a multiple of the tile."""
+    64-cell row segments, in FP32) against the oracle network on features built on the host,
+    on rows shorter than a segment, rows that end inside a segment, and whole segments."""
     from helpers import smooth_random_state
-    shape = (20, 7, 5)
     nx, ny, nz = shape
     params, _, _ = make(13, B=16)
     scales = np.array([3.0, 4.0, 4.0, 4.0, 5.0, 0.2, 0.2, 0.2, 0.1], np.float32)
