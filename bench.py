@@ -42,7 +42,7 @@ BYTES_PER_LUP = {("D3Q27", "fp64"): 2 * (27 * 8 + 5 * 8) + 4, ("D3Q27", "split")
                  ("D3Q125", "fp64"): 2 * (125 * 8 + 5 * 8) + 4, ("D3Q125", "split"): 2 * (125 * 4 + 4 * 4 + 8) + 4,
                  ("D1Q5", "fp64"): 2 * (5 * 8 + 5 * 8) + 4}
 # the step kernel each path launches for the bench configurations
-KERNEL_NAME = {("D3Q27", "fp64"): "k_step_f64<1,2,true>", ("D3Q27", "split"): "k_step_split27<2>",
+KERNEL_NAME = {("D3Q27", "fp64"): "k_step_f64<1,2,true>", ("D3Q27", "split"): "k_step_split27_refill<2>",
                ("D3Q125", "fp64"): "k_step_q125_f64", ("D3Q125", "split"): "k_step_split125"}
 # algorithmic FP64 flops per cell-iteration of the reference's algorithm with de-duplicated
 # transfers: 19 flops per D3Q27 transfer term, 27 targets x 50 transfers x 27 terms (WENO5)
@@ -140,7 +140,14 @@ def cpu_baseline(case_name, interp):
     dt = time.perf_counter() - t
     cells = int(np.prod(case.shape))
     niter = float(o.fields()["niter"].mean())
+    cpu = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            cpu = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), cpu)
+    except OSError:
+        pass
     return {"value": round(cells / dt / 1e6, 6), "unit": "MLUPS", "cores": pyoracle.threads(), "kind": "port",
+            "cpu": cpu,
             "sample": f"oracle (reference algorithm as written, no memoisation), {case.lattice} {interp} "
                       f"{sample_n}^3 cells x 1 step of the {case_name} initial-condition family "
                       f"({cells} cell-steps in {dt:.1f} s, mean gauge iterations {niter:.2f}, "
