@@ -328,9 +328,9 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
         return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of the split state buffers failed");
       ctx->bytes += (int64_t)(fbytes + mbytes + tbytes);
     }
-    if (hipMalloc(&ctx->sq, (size_t)g.plane * sizeof(float)) != hipSuccess)
-      return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of the sqrt(T) plane failed");
-    ctx->bytes += (int64_t)g.plane * sizeof(float);
+    if (hipMalloc(&ctx->sq, (size_t)g.plane * 4 * sizeof(float)) != hipSuccess)
+      return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of the node gauge records failed");
+    ctx->bytes += (int64_t)g.plane * 4 * sizeof(float);
     if (p->use_mlp) {
       if (hipMalloc(&ctx->guess, (size_t)4 * g.plane * sizeof(float)) != hipSuccess)
         return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of the predicted-gauge planes failed");
@@ -533,8 +533,8 @@ pond_status pond_step(pond_ctx* ctx, int nsteps) {
     const int in = ctx->cur, outb = 1 - ctx->cur;
     hipError_t he = hipSuccess;
     if (ctx->split) {
-      he = launch_sqrt_t(ctx->sp[in].T, ctx->sq, ctx->g.n, ctx->stream);
-      if (he != hipSuccess) return fail(ctx, POND_ERR_HIP, std::string("sqrt(T) kernel: ") + hipGetErrorString(he));
+      he = launch_node_gauge(ctx->sp[in].m, ctx->sp[in].T, ctx->sq, ctx->g.n, ctx->g.plane, ctx->stream);
+      if (he != hipSuccess) return fail(ctx, POND_ERR_HIP, std::string("node gauge kernel: ") + hipGetErrorString(he));
     }
     const bool learned = ctx->split && ctx->p.use_mlp && ctx->mlp.trained;
     if (learned) {

@@ -79,7 +79,7 @@ struct pond_ctx {
   pond::SplitArgs sargs{};
   double* nodeP = nullptr;  // plain path: sqrt(|T|)*sign(T) per cell of the load buffer
   unsigned char* nodeok = nullptr;  // plain path: populations finite and below 2^200
-  float* sq = nullptr;      // split path: sqrt(|T|)*sign(T) per cell of the load buffer
+  float* sq = nullptr;      // split path: node gauge records (ux, uy, uz, sqrt(|T|)*sign(T)) of the load buffer
   float* guess = nullptr;   // split path: learned starting gauge increments (4 planes)
   // cells left unconverged by the first pass of a two-pass step: count, then per entry the
   // cell index, 4 gauge doubles, one double and one int (48 B per cell of the grid)

@@ -29,8 +29,9 @@ struct SplitPtrs {
 
 using SplitPending = Pending;
 
-hipError_t launch_sqrt_t(const double* T, float* sq, int64_t n, hipStream_t st);
-hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const SplitPtrs& out, const float* sq,
+// node gauge records (ux, uy, uz, sqrt(|T|)*sign(T)), 16 bytes per cell, of a split state
+hipError_t launch_node_gauge(const float* m, const double* T, float* g4, int64_t n, int64_t plane, hipStream_t st);
+hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const SplitPtrs& out, const float* g4,
                              int* niter, const float* guess, const SplitPending& pend, const SplitArgs& a,
                              hipStream_t st);
 

@@ -142,19 +142,34 @@ __device__ __forceinline__ float transfer27f(const PlanesF& f, uint32_t boff, fl
   return out;
 }
 
-// node side of the load state: populations, (rho, ux, uy, uz) and sqrt(|T|)*sign(T) planes
+// node side of the load state: the population planes and the node gauges, one 16-byte record
+// (ux, uy, uz, sqrt(|T|)*sign(T)) per cell written by k_node_gauge before the step, so a node's
+// gauge is one load
 struct NodeSrcF {
-  PlanesF f, m, sq;
+  PlanesF f;
+  __amdgpu_buffer_rsrc_t g4;
   Grid g;
 };
 
-__device__ __forceinline__ NodeSrcF make_src(const float* f, const float* m, const float* sq, const Grid& g) {
-  return NodeSrcF{make_planes(f, 27, g.plane), make_planes(m, 4, g.plane), make_planes(sq, 1, g.plane), g};
+__device__ __forceinline__ NodeSrcF make_src(const float* f, const float* g4, const Grid& g) {
+  return NodeSrcF{make_planes(f, 27, g.plane), make_rsrc(g4, (uint64_t)g.plane * 16u), g};
+}
+
+struct Gauge4 {
+  float ux, uy, uz, sq;
+};
+
+// gauge record of the cell at byte offset boff of the population planes (4 bytes per cell)
+__device__ __forceinline__ Gauge4 ldg4(const NodeSrcF& s, uint32_t boff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(s.g4, (int)(boff * 4u), 0, 0);
+  return Gauge4{__builtin_bit_cast(float, (uint32_t)v[0]), __builtin_bit_cast(float, (uint32_t)v[1]),
+                __builtin_bit_cast(float, (uint32_t)v[2]), __builtin_bit_cast(float, (uint32_t)v[3])};
 }
 
 __device__ __forceinline__ float node_xfer(const NodeSrcF& s, int64_t c, const TgtF& t, float sqTb) {
   const uint32_t boff = (uint32_t)c * 4u;
-  return transfer27f(s.f, boff, ldf(s.m, 1, boff), ldf(s.m, 2, boff), ldf(s.m, 3, boff), ldf(s.sq, 0, boff), t, sqTb);
+  const Gauge4 n = ldg4(s, boff);
+  return transfer27f(s.f, boff, n.ux, n.uy, n.uz, n.sq, t, sqTb);
 }
 
 __device__ __forceinline__ void lag3f(float s, float F[3]) {
@@ -390,30 +405,27 @@ __device__ void interp_group_weno5c(const NodeSrcF& src, int x, int y, int z, do
         float f[27];
 #pragma unroll
         for (int k = 0; k < 27; k++) f[k] = ldf(src.f, k, boff);
-        node(ldf(src.m, 1, boff), ldf(src.m, 2, boff), ldf(src.m, 3, boff), ldf(src.sq, 0, boff), f, wab, q == 4, yz,
-             mid);
+        const Gauge4 n = ldg4(src, boff);
+        node(n.ux, n.uy, n.uz, n.sq, f, wab, q == 4, yz, mid);
       } else if (paired) {
-        float f[27], f2[27], mx[2], my[2], mz[2], sq[2];
+        float f[27], f2[27];
 #pragma unroll
         for (int k = 0; k < 27; k++) ldf2(src.f, k, boff, f[k], f2[k]);
-        ldf2(src.m, 1, boff, mx[0], mx[1]);
-        ldf2(src.m, 2, boff, my[0], my[1]);
-        ldf2(src.m, 3, boff, mz[0], mz[1]);
-        ldf2(src.sq, 0, boff, sq[0], sq[1]);
-        node(mx[0], my[0], mz[0], sq[0], f, wab, q == 4, yz, mid);
-        if (two) node(mx[1], my[1], mz[1], sq[1], f2, wab, q == 4, yz2, mid2);
+        const Gauge4 n = ldg4(src, boff), n2 = ldg4(src, boff + 4u);
+        node(n.ux, n.uy, n.uz, n.sq, f, wab, q == 4, yz, mid);
+        if (two) node(n2.ux, n2.uy, n2.uz, n2.sq, f2, wab, q == 4, yz2, mid2);
       } else {
         float f[27];
 #pragma unroll
         for (int k = 0; k < 27; k++) f[k] = ldf(src.f, k, boff);
-        node(ldf(src.m, 1, boff), ldf(src.m, 2, boff), ldf(src.m, 3, boff), ldf(src.sq, 0, boff), f, wab, q == 4, yz,
-             mid);
+        const Gauge4 n = ldg4(src, boff);
+        node(n.ux, n.uy, n.uz, n.sq, f, wab, q == 4, yz, mid);
         if (two) {
           const uint32_t boff2 = (uint32_t)cell_index(g, xw2, ya, zb) * 4u;
 #pragma unroll
           for (int k = 0; k < 27; k++) f[k] = ldf(src.f, k, boff2);
-          node(ldf(src.m, 1, boff2), ldf(src.m, 2, boff2), ldf(src.m, 3, boff2), ldf(src.sq, 0, boff2), f, wab, q == 4,
-               yz2, mid2);
+          const Gauge4 n2 = ldg4(src, boff2);
+          node(n2.ux, n2.uy, n2.uz, n2.sq, f, wab, q == 4, yz2, mid2);
         }
       }
     }
@@ -431,12 +443,19 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
   s = t;
 }
 
-// sqrt(|T|)*sign(T) of every cell, in FP32, for the node side of the transfer
-__global__ void k_sqrt_t(const double* __restrict__ T, float* __restrict__ sq, int64_t n) {
+// node gauge record of every cell for the node side of the transfer: (ux, uy, uz) and
+// sqrt(|T|)*sign(T) in FP32, 16 bytes per cell
+__global__ void k_node_gauge(const float* __restrict__ m, const double* __restrict__ T, float* __restrict__ g4,
+                             int64_t n, int64_t plane) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const double t = T[c];
-  sq[c] = (float)(sqrt(fabs(t)) * (double)((0.0 < t) - (t < 0.0)));
+  float4 v;
+  v.x = m[plane + c];
+  v.y = m[2 * plane + c];
+  v.z = m[3 * plane + c];
+  v.w = (float)(sqrt(fabs(t)) * (double)((0.0 < t) - (t < 0.0)));
+  reinterpret_cast<float4*>(g4)[c] = v;
 }
 
 // iterations a predicted starting gauge gets to converge before the cell falls back to its
@@ -641,7 +660,7 @@ __device__ __forceinline__ void split27_start(const float* __restrict__ min, con
 // with one x slice at a time.
 template <int INTERP>
 __device__ __forceinline__ void split27_pass1(const float* __restrict__ fin, const float* __restrict__ min,
-                                              const double* __restrict__ Tin, const float* __restrict__ sqin,
+                                              const double* __restrict__ Tin, const float* __restrict__ g4in,
                                               float* __restrict__ fout, float* __restrict__ mout,
                                               double* __restrict__ Tout, int* __restrict__ niter_out,
                                               const float* __restrict__ guess, const SplitPending& pend,
@@ -655,7 +674,7 @@ __device__ __forceinline__ void split27_pass1(const float* __restrict__ fin, con
   if (live) {
     double gp[4];
     split27_start(min, Tin, guess, cell, gp, st, a);
-    const NodeSrcF src = make_src(fin, min, sqin, g);
+    const NodeSrcF src = make_src(fin, g4in, g);
     const bool conv = split27_iterate<INTERP>(src, fout, cell, x, y, z, gp, st, a.pass1_iters, a);
     pending = !conv && st.niter - st.base < a.max_iter;
     if (!pending) split27_finish(fout, mout, Tout, niter_out, cell, st, a);
@@ -673,20 +692,20 @@ __device__ __forceinline__ void split27_pass1(const float* __restrict__ fin, con
 
 template <int INTERP>
 __global__ __launch_bounds__(64) void k_step_split27(const float* __restrict__ fin, const float* __restrict__ min,
-                                                     const double* __restrict__ Tin, const float* __restrict__ sqin,
+                                                     const double* __restrict__ Tin, const float* __restrict__ g4in,
                                                      float* __restrict__ fout, float* __restrict__ mout,
                                                      double* __restrict__ Tout, int* __restrict__ niter_out,
                                                      const float* __restrict__ guess, SplitPending pend,
                                                      const SplitArgs a) {
-  split27_pass1<INTERP>(fin, min, Tin, sqin, fout, mout, Tout, niter_out, guess, pend, a);
+  split27_pass1<INTERP>(fin, min, Tin, g4in, fout, mout, Tout, niter_out, guess, pend, a);
 }
 
 // the grouped WENO5C step held to 168 VGPRs (3 waves per SIMD; a few spilled registers)
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_step_split27_g3(
     const float* __restrict__ fin, const float* __restrict__ min, const double* __restrict__ Tin,
-    const float* __restrict__ sqin, float* __restrict__ fout, float* __restrict__ mout, double* __restrict__ Tout,
+    const float* __restrict__ g4in, float* __restrict__ fout, float* __restrict__ mout, double* __restrict__ Tout,
     int* __restrict__ niter_out, const float* __restrict__ guess, SplitPending pend, const SplitArgs a) {
-  split27_pass1<2>(fin, min, Tin, sqin, fout, mout, Tout, niter_out, guess, pend, a);
+  split27_pass1<2>(fin, min, Tin, g4in, fout, mout, Tout, niter_out, guess, pend, a);
 }
 
 // Single pass with lane refill.  A wave owns a chunk of `chunk` consecutive 64-cell tiles.  Its
@@ -700,7 +719,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
 template <int INTERP>
 __global__ __launch_bounds__(64) void k_step_split27_refill(
     const float* __restrict__ fin, const float* __restrict__ min, const double* __restrict__ Tin,
-    const float* __restrict__ sqin, float* __restrict__ fout, float* __restrict__ mout, double* __restrict__ Tout,
+    const float* __restrict__ g4in, float* __restrict__ fout, float* __restrict__ mout, double* __restrict__ Tout,
     int* __restrict__ niter_out, const float* __restrict__ guess, const SplitArgs a, int chunk) {
   const Grid& g = a.g;
   const int lane = threadIdx.x;
@@ -708,7 +727,7 @@ __global__ __launch_bounds__(64) void k_step_split27_refill(
   const int64_t w = xcd_block(blockIdx.x, g.xcd ? (int64_t)gridDim.x : -1);
   const int64_t t0 = w * chunk;
   const int64_t total = (ntiles - t0 < chunk ? ntiles - t0 : chunk) * 64;  // cell slots of the chunk
-  const NodeSrcF src = make_src(fin, min, sqin, g);
+  const NodeSrcF src = make_src(fin, g4in, g);
   int64_t next = 0;  // first unstarted slot (the same in every lane)
   bool active = false;
   int64_t cell = 0;
@@ -746,13 +765,13 @@ __global__ __launch_bounds__(64) void k_step_split27_refill(
 // where the first pass left it.
 template <int INTERP>
 __global__ __launch_bounds__(64) void k_step_split27_rest(const float* __restrict__ fin, const float* __restrict__ min,
-                                                          const double* __restrict__ Tin, const float* __restrict__ sqin,
+                                                          const double* __restrict__ Tin, const float* __restrict__ g4in,
                                                           float* __restrict__ fout, float* __restrict__ mout,
                                                           double* __restrict__ Tout, int* __restrict__ niter_out,
                                                           SplitPending pend, const SplitArgs a) {
   const Grid& g = a.g;
   const int count = *pend.count;
-  const NodeSrcF src = make_src(fin, min, sqin, g);
+  const NodeSrcF src = make_src(fin, g4in, g);
   for (int64_t e = (int64_t)blockIdx.x * 64 + threadIdx.x; e - threadIdx.x < count; e += (int64_t)gridDim.x * 64) {
     if (e >= count) continue;
     const int64_t cell = pend.cell[e];
@@ -986,12 +1005,12 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
   }
 }
 
-hipError_t launch_sqrt_t(const double* T, float* sq, int64_t n, hipStream_t st) {
-  hipLaunchKernelGGL(k_sqrt_t, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, T, sq, n);
+hipError_t launch_node_gauge(const float* m, const double* T, float* g4, int64_t n, int64_t plane, hipStream_t st) {
+  hipLaunchKernelGGL(k_node_gauge, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, m, T, g4, n, plane);
   return hipGetLastError();
 }
 
-hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const SplitPtrs& out, const float* sq,
+hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const SplitPtrs& out, const float* g4,
                              int* niter, const float* guess, const SplitPending& pend, const SplitArgs& a,
                              hipStream_t st) {
   if (lattice == kD3Q125) {
@@ -1007,14 +1026,14 @@ hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const
   const unsigned blocks2 = (unsigned)(w2 < 16384 ? w2 : 16384);
 #define POND_LAUNCH_S(K, I)                                                                                       \
   do {                                                                                                            \
-    hipLaunchKernelGGL((K), dim3(blocks), dim3(64), 0, st, in.f, in.m, in.T, sq, out.f, out.m, out.T, niter, guess, \
+    hipLaunchKernelGGL((K), dim3(blocks), dim3(64), 0, st, in.f, in.m, in.T, g4, out.f, out.m, out.T, niter, guess, \
                        pend, a);                                                                                  \
-    hipLaunchKernelGGL((k_step_split27_rest<I>), dim3(blocks2), dim3(64), 0, st, in.f, in.m, in.T, sq, out.f,     \
+    hipLaunchKernelGGL((k_step_split27_rest<I>), dim3(blocks2), dim3(64), 0, st, in.f, in.m, in.T, g4, out.f,     \
                        out.m, out.T, niter, pend, a);                                                             \
   } while (0)
   if (interp == 2 && a.weno5c_variant == 0 && a.refill_chunk > 0 && a.pass1_iters >= a.max_iter + kLearnedBudget) {
     const int64_t waves = ((int64_t)blocks + a.refill_chunk - 1) / a.refill_chunk;
-    hipLaunchKernelGGL(k_step_split27_refill<2>, dim3((unsigned)waves), dim3(64), 0, st, in.f, in.m, in.T, sq, out.f,
+    hipLaunchKernelGGL(k_step_split27_refill<2>, dim3((unsigned)waves), dim3(64), 0, st, in.f, in.m, in.T, g4, out.f,
                        out.m, out.T, niter, guess, a, a.refill_chunk);
     return hipGetLastError();
   }
