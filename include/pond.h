@@ -59,7 +59,7 @@ typedef enum { POND_WENO5 = 0, POND_LAGR4 = 1, POND_WENO5C = 2 } pond_interp;
 /* POND_FP64: plain path, reference operation order, FP64 everywhere.
    POND_SPLIT: FP32 populations/streaming/interpolation/collide, compensated moment sums,
    FP64 gauge residual, temperature/energy moment and moment solve.  D3Q27 (any
-   interpolator, up to 119,304,640 cells) and D3Q125 (its fixed 3x3x3 stencil). */
+   interpolator, up to 134,217,664 cells) and D3Q125 (its fixed 3x3x3 stencil). */
 typedef enum { POND_FP64 = 0, POND_SPLIT = 1 } pond_precision;
 
 typedef struct {

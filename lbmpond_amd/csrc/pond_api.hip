@@ -45,11 +45,11 @@ __global__ void k_set_base(FT* __restrict__ f, const double* __restrict__ rho, i
 
 // split layout from FP64 moment planes: f_i = w_i*rho in FP32, (rho,u) in FP32, T in FP64
 __global__ void k_set_split(const double* __restrict__ m, float* __restrict__ f, float* __restrict__ mf,
-                            double* __restrict__ T, int64_t n, int64_t plane, int qn, const StepArgs a) {
+                            double* __restrict__ T, int64_t n, int64_t plane, int lattice, int qn, const StepArgs a) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const double r = m[c];
-  for (int i = 0; i < qn; i++) f[(int64_t)i * plane + c] = (float)(a.w[i] * r);
+  for (int i = 0; i < qn; i++) f[split_fidx(lattice, i, c, plane)] = (float)(a.w[i] * r);
   for (int k = 0; k < 4; k++) mf[(int64_t)k * plane + c] = (float)m[(int64_t)k * plane + c];
   T[c] = m[(int64_t)4 * plane + c];
 }
@@ -270,8 +270,10 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
     g.ntz = (g.nz + g.tz - 1) / g.tz;
   }
   // the split step addresses nine FP32 population planes through one 32-bit buffer offset
-  if (p->precision == POND_SPLIT && p->lattice == POND_D3Q27 && g.plane * 4 * 9 > (int64_t)UINT32_MAX)
-    return bail(POND_ERR_INVALID, "pond_create: POND_SPLIT D3Q27 supports up to 119,304,640 cells (about 492^3)");
+  // the split step addresses two population quad planes (16 bytes per cell each) through one
+  // 32-bit buffer offset
+  if (p->precision == POND_SPLIT && p->lattice == POND_D3Q27 && g.plane * 32 > (int64_t)UINT32_MAX)
+    return bail(POND_ERR_INVALID, "pond_create: POND_SPLIT D3Q27 supports up to 134,217,664 cells (about 512^3)");
   StepArgs& a = ctx->args;
   a.g = g;
   a.dtau = p->dtau;
@@ -319,7 +321,7 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
       return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of the node planes failed");
     ctx->bytes += (int64_t)g.plane * 9;
   } else {
-    const size_t fbytes = (size_t)ctx->Qn * g.plane * sizeof(float);
+    const size_t fbytes = (size_t)split_fcount(p->lattice, ctx->Qn, g.plane) * sizeof(float);
     const size_t mbytes = (size_t)4 * g.plane * sizeof(float);
     const size_t tbytes = (size_t)g.plane * sizeof(double);
     for (int b = 0; b < 2; b++) {
@@ -427,14 +429,15 @@ pond_status pond_set_state(pond_ctx* ctx, const double* rho, const double* ux, c
     pond_status s = upload_moments(ctx, tmp, rho, ux, uy, uz, T);
     if (s == POND_OK) {
       hipLaunchKernelGGL(k_set_split, dim3(blocks), dim3(256), 0, ctx->stream, tmp, ctx->sp[0].f, ctx->sp[0].m,
-                         ctx->sp[0].T, g.n, g.plane, ctx->Qn, ctx->args);
+                         ctx->sp[0].T, g.n, g.plane, ctx->p.lattice, ctx->Qn, ctx->args);
       hipError_t e = hipGetLastError();
       if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
       if (e != hipSuccess) s = fail(ctx, POND_ERR_HIP, std::string("k_set_split: ") + hipGetErrorString(e));
     }
     (void)hipFree(tmp);
     if (s != POND_OK) return s;
-    POND_HIP(ctx, hipMemcpyAsync(ctx->sp[1].f, ctx->sp[0].f, (size_t)ctx->Qn * g.plane * sizeof(float),
+    POND_HIP(ctx, hipMemcpyAsync(ctx->sp[1].f, ctx->sp[0].f,
+                                 (size_t)split_fcount(ctx->p.lattice, ctx->Qn, g.plane) * sizeof(float),
                                  hipMemcpyDeviceToDevice, ctx->stream));
     POND_HIP(ctx, hipMemcpyAsync(ctx->sp[1].m, ctx->sp[0].m, (size_t)4 * g.plane * sizeof(float),
                                  hipMemcpyDeviceToDevice, ctx->stream));
@@ -472,13 +475,13 @@ pond_status pond_set_populations(pond_ctx* ctx, const double* f, const double* r
     const double* src[5] = {rho, ux, uy, uz, T};
     for (int k = 0; k < 5; k++)
       if (!src[k]) return fail(ctx, POND_ERR_INVALID, "null field pointer");
+    const int64_t fcount = split_fcount(ctx->p.lattice, ctx->Qn, g.plane);
+    std::vector<float> fs((size_t)fcount, 0.0f);
+    for (int i = 0; i < ctx->Qn; i++)
+      for (int64_t c = 0; c < g.n; c++) fs[split_fidx(ctx->p.lattice, i, c, g.plane)] = (float)f[(size_t)i * g.n + c];
     std::vector<float> tmp((size_t)g.n);
     for (int b = 0; b < 2; b++) {
-      for (int i = 0; i < ctx->Qn; i++) {
-        for (int64_t c = 0; c < g.n; c++) tmp[c] = (float)f[(size_t)i * g.n + c];
-        POND_HIP(ctx, hipMemcpy(ctx->sp[b].f + (size_t)i * g.plane, tmp.data(), (size_t)g.n * sizeof(float),
-                                hipMemcpyHostToDevice));
-      }
+      POND_HIP(ctx, hipMemcpy(ctx->sp[b].f, fs.data(), (size_t)fcount * sizeof(float), hipMemcpyHostToDevice));
       for (int k = 0; k < 4; k++) {
         for (int64_t c = 0; c < g.n; c++) tmp[c] = (float)src[k][c];
         POND_HIP(ctx, hipMemcpy(ctx->sp[b].m + (size_t)k * g.plane, tmp.data(), (size_t)g.n * sizeof(float),
@@ -611,12 +614,11 @@ pond_status pond_get_populations(pond_ctx* ctx, double* f) {
   if (s != POND_OK) return s;
   const Grid& g = ctx->g;
   if (ctx->split) {
-    std::vector<float> tmp((size_t)g.n);
-    for (int i = 0; i < ctx->Qn; i++) {
-      POND_HIP(ctx, hipMemcpy(tmp.data(), ctx->sp[ctx->cur].f + (size_t)i * g.plane, (size_t)g.n * sizeof(float),
-                              hipMemcpyDeviceToHost));
-      for (int64_t c = 0; c < g.n; c++) f[(size_t)i * g.n + c] = tmp[c];
-    }
+    const int64_t fcount = split_fcount(ctx->p.lattice, ctx->Qn, g.plane);
+    std::vector<float> fs((size_t)fcount);
+    POND_HIP(ctx, hipMemcpy(fs.data(), ctx->sp[ctx->cur].f, (size_t)fcount * sizeof(float), hipMemcpyDeviceToHost));
+    for (int i = 0; i < ctx->Qn; i++)
+      for (int64_t c = 0; c < g.n; c++) f[(size_t)i * g.n + c] = fs[split_fidx(ctx->p.lattice, i, c, g.plane)];
     return POND_OK;
   }
   for (int i = 0; i < ctx->Qn; i++)

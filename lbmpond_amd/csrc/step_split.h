@@ -21,8 +21,24 @@ struct SplitArgs {
   float w[kQmax];
 };
 
+// D3Q27 split populations are stored in seven quad planes: population k of cell c at float index
+// ((k / 4) * plane + c) * 4 + k % 4 (16 bytes per cell and quad plane, the 28th slot unused), so
+// a stencil node's 27 populations are seven 16-byte loads.  D3Q125 keeps one plane per
+// population (index k * plane + c).
+constexpr int kSplit27Floats = 28;  // floats per cell of the D3Q27 split population storage
+__host__ __device__ inline int64_t split27_fidx(int k, int64_t c, int64_t plane) {
+  return (((int64_t)(k >> 2) * plane + c) << 2) + (k & 3);
+}
+__host__ __device__ inline int64_t split_fidx(int lattice, int k, int64_t c, int64_t plane) {
+  return lattice == kD3Q27 ? split27_fidx(k, c, plane) : (int64_t)k * plane + c;
+}
+// floats of one split population buffer
+inline int64_t split_fcount(int lattice, int qn, int64_t plane) {
+  return (lattice == kD3Q27 ? kSplit27Floats : qn) * plane;
+}
+
 struct SplitPtrs {
-  float* f;   // Qn planes
+  float* f;   // populations (layout above)
   float* m;   // rho, ux, uy, uz planes
   double* T;  // temperature plane
 };

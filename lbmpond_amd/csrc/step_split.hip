@@ -73,46 +73,44 @@ __device__ __forceinline__ void basis3(float D, float sq0, bool kn, float c, flo
   }
 }
 
-// Population planes as buffer resources: resource j covers planes 9j .. 9j+8, a load names its
-// plane through the scalar offset (k mod 9) * plane bytes, and the per-lane address is a single
-// 32-bit offset, where a global load needs a 64-bit per-lane address (two VGPRs and a 64-bit
-// add per load).  The resource size is exact, so the hardware range check returns 0 rather
-// than reading past the array.  Word 3 (0x00020000) is the raw-buffer configuration of the gfx9
-// family.  Usable while 9 planes fit in 4 GiB (plane <= 119,304,647 cells).
-constexpr int kPlanesPerRsrc = 9;
+// Buffer resources: a load names its plane (or quad plane) through the scalar offset and the
+// per-lane address is a single 32-bit offset, where a global load needs a 64-bit per-lane
+// address (two VGPRs and a 64-bit add per load).  The resource size is exact, so the hardware
+// range check returns 0 rather than reading past the array.  Word 3 (0x00020000) is the
+// raw-buffer configuration of the gfx9 family.
 constexpr uint32_t kRawBufferWord3 = 0x00020000u;
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint64_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)(uint32_t)bytes, (int)kRawBufferWord3);
 }
 
-struct PlanesF {
-  __amdgpu_buffer_rsrc_t r[3];
-  uint32_t pbytes;  // bytes per plane
+// the seven quad planes of the D3Q27 split populations: resource j holds quad planes 2j, 2j+1,
+// the quad plane within it is chosen by the scalar offset (plane * 16 bytes each, so up to
+// 134,217,727 cells fit the 32-bit offsets)
+struct QuadsF {
+  __amdgpu_buffer_rsrc_t r[4];
+  uint32_t qbytes;  // bytes per quad plane
 };
 
-__device__ __forceinline__ PlanesF make_planes(const float* base, int nplanes, int64_t plane) {
-  PlanesF p;
-  p.pbytes = (uint32_t)(plane * 4);
-  for (int j = 0; j < 3; j++) {
-    const int first = j * kPlanesPerRsrc;
-    const int cnt = nplanes - first < kPlanesPerRsrc ? nplanes - first : kPlanesPerRsrc;
-    p.r[j] = make_rsrc(base + (int64_t)(first < nplanes ? first : 0) * plane, cnt > 0 ? (uint64_t)cnt * p.pbytes : 0);
+__device__ __forceinline__ QuadsF make_quads(const float* base, int64_t plane) {
+  QuadsF q;
+  q.qbytes = (uint32_t)(plane * 16);
+  for (int j = 0; j < 4; j++) {
+    const int cnt = 7 - 2 * j < 2 ? 7 - 2 * j : 2;
+    q.r[j] = make_rsrc(base + (int64_t)2 * j * plane * 4, (uint64_t)cnt * q.qbytes);
   }
-  return p;
+  return q;
 }
 
-__device__ __forceinline__ float ldf(const PlanesF& p, int k, uint32_t boff) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(p.r[k / kPlanesPerRsrc], (int)boff,
-                                                                        (int)((uint32_t)(k % kPlanesPerRsrc) * p.pbytes), 0));
-}
-
-// plane k of two neighbouring cells at byte offsets boff and boff + 4 in one 8-byte load
-__device__ __forceinline__ void ldf2(const PlanesF& p, int k, uint32_t boff, float& a, float& b) {
-  const auto v = __builtin_amdgcn_raw_buffer_load_b64(p.r[k / kPlanesPerRsrc], (int)boff,
-                                                      (int)((uint32_t)(k % kPlanesPerRsrc) * p.pbytes), 0);
-  a = __builtin_bit_cast(float, (uint32_t)v[0]);
-  b = __builtin_bit_cast(float, (uint32_t)v[1]);
+// the 27 populations of the cell at byte offset boff of a 4-byte plane (boff = 4 * cell)
+__device__ __forceinline__ void load27(const QuadsF& q, uint32_t boff, float f[27]) {
+#pragma unroll
+  for (int j = 0; j < 7; j++) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(q.r[j >> 1], (int)(boff * 4u), (int)((uint32_t)(j & 1) * q.qbytes), 0);
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+      if (4 * j + e < 27) f[4 * j + e] = __builtin_bit_cast(float, (uint32_t)v[e]);
+  }
 }
 
 // sum-factorised gauge transfer of the node at byte offset boff to target t.  The gauge
@@ -120,8 +118,8 @@ __device__ __forceinline__ void ldf2(const PlanesF& p, int k, uint32_t boff, flo
 // within a factor of two (Sterbenz), so the difference carries one rounding relative to itself
 // rather than one relative to |u1|, which would put a jitter of ulp(|u|) into the velocity
 // moment.
-__device__ __forceinline__ float transfer27f(const PlanesF& f, uint32_t boff, float u0x, float u0y, float u0z,
-                                             float sq0, const TgtF& t, float sqTb) {
+__device__ __forceinline__ float transfer27f(const float f[27], float u0x, float u0y, float u0z, float sq0,
+                                             const TgtF& t, float sqTb) {
   float Lx[3], Ly[3], Lz[3];
   basis3(((t.u1x - u0x) + t.u1xl) * sqTb, sq0, t.kx, t.cx, t.T1, t.wx, Lx);
   basis3(((t.u1y - u0y) + t.u1yl) * sqTb, sq0, t.ky, t.cy, t.T1, t.wy, Ly);
@@ -134,7 +132,7 @@ __device__ __forceinline__ float transfer27f(const PlanesF& f, uint32_t boff, fl
     for (int b = 0; b < 3; b++) {
       float sz = 0.0f;
 #pragma unroll
-      for (int c = 0; c < 3; c++) sz = __builtin_fmaf(Lz[c], ldf(f, kIdx27.v[a][b][c], boff), sz);
+      for (int c = 0; c < 3; c++) sz = __builtin_fmaf(Lz[c], f[kIdx27.v[a][b][c]], sz);
       sy = __builtin_fmaf(Ly[b], sz, sy);
     }
     out = __builtin_fmaf(Lx[a], sy, out);
@@ -146,13 +144,13 @@ __device__ __forceinline__ float transfer27f(const PlanesF& f, uint32_t boff, fl
 // (ux, uy, uz, sqrt(|T|)*sign(T)) per cell written by k_node_gauge before the step, so a node's
 // gauge is one load
 struct NodeSrcF {
-  PlanesF f;
+  QuadsF f;
   __amdgpu_buffer_rsrc_t g4;
   Grid g;
 };
 
 __device__ __forceinline__ NodeSrcF make_src(const float* f, const float* g4, const Grid& g) {
-  return NodeSrcF{make_planes(f, 27, g.plane), make_rsrc(g4, (uint64_t)g.plane * 16u), g};
+  return NodeSrcF{make_quads(f, g.plane), make_rsrc(g4, (uint64_t)g.plane * 16u), g};
 }
 
 struct Gauge4 {
@@ -168,8 +166,10 @@ __device__ __forceinline__ Gauge4 ldg4(const NodeSrcF& s, uint32_t boff) {
 
 __device__ __forceinline__ float node_xfer(const NodeSrcF& s, int64_t c, const TgtF& t, float sqTb) {
   const uint32_t boff = (uint32_t)c * 4u;
+  float f[27];
+  load27(s.f, boff, f);
   const Gauge4 n = ldg4(s, boff);
-  return transfer27f(s.f, boff, n.ux, n.uy, n.uz, n.sq, t, sqTb);
+  return transfer27f(f, n.ux, n.uy, n.uz, n.sq, t, sqTb);
 }
 
 __device__ __forceinline__ void lag3f(float s, float F[3]) {
@@ -385,12 +385,9 @@ __device__ void interp_group_weno5c(const NodeSrcF& src, int x, int y, int z, do
     const int xw = wrapi(xs, g.nx);
     float yz[3] = {0.f, 0.f, 0.f}, mid[3] = {0.f, 0.f, 0.f};
     float yz2[3] = {0.f, 0.f, 0.f}, mid2[3] = {0.f, 0.f, 0.f};
-    // PAIRS: slices xs and xs+1 together, each node and plane of the two by one 8-byte load
-    // where xs+1 is the next cell of the row in every lane of the wave (away from the periodic
-    // wrap); a lane whose window ends at xs loads the pair and ignores its second half
+    // PAIRS: slices xs and xs+1 together (the second through the same per-node code)
     const bool two = PAIRS && xs + 1 <= xhi;
     const int xw2 = PAIRS ? wrapi(xs + 1, g.nx) : 0;
-    const bool paired = PAIRS && __all(!two || xw2 == xw + 1);
     // one node at a time: its loads are the only ones in flight per lane
 #pragma unroll 1
     for (int q = 0; q < 9; q++) {
@@ -401,32 +398,15 @@ __device__ void interp_group_weno5c(const NodeSrcF& src, int x, int y, int z, do
       const float fzb = b == 0 ? Fz[0] : b == 1 ? Fz[1] : Fz[2];
       const float wab = fya * fzb;
       const uint32_t boff = (uint32_t)cell_index(g, xw, ya, zb) * 4u;
-      if (!PAIRS) {
-        float f[27];
-#pragma unroll
-        for (int k = 0; k < 27; k++) f[k] = ldf(src.f, k, boff);
-        const Gauge4 n = ldg4(src, boff);
-        node(n.ux, n.uy, n.uz, n.sq, f, wab, q == 4, yz, mid);
-      } else if (paired) {
-        float f[27], f2[27];
-#pragma unroll
-        for (int k = 0; k < 27; k++) ldf2(src.f, k, boff, f[k], f2[k]);
-        const Gauge4 n = ldg4(src, boff), n2 = ldg4(src, boff + 4u);
-        node(n.ux, n.uy, n.uz, n.sq, f, wab, q == 4, yz, mid);
-        if (two) node(n2.ux, n2.uy, n2.uz, n2.sq, f2, wab, q == 4, yz2, mid2);
-      } else {
-        float f[27];
-#pragma unroll
-        for (int k = 0; k < 27; k++) f[k] = ldf(src.f, k, boff);
-        const Gauge4 n = ldg4(src, boff);
-        node(n.ux, n.uy, n.uz, n.sq, f, wab, q == 4, yz, mid);
-        if (two) {
-          const uint32_t boff2 = (uint32_t)cell_index(g, xw2, ya, zb) * 4u;
-#pragma unroll
-          for (int k = 0; k < 27; k++) f[k] = ldf(src.f, k, boff2);
-          const Gauge4 n2 = ldg4(src, boff2);
-          node(n2.ux, n2.uy, n2.uz, n2.sq, f, wab, q == 4, yz2, mid2);
-        }
+      float f[27];
+      load27(src.f, boff, f);
+      const Gauge4 n = ldg4(src, boff);
+      node(n.ux, n.uy, n.uz, n.sq, f, wab, q == 4, yz, mid);
+      if (two) {
+        const uint32_t boff2 = (uint32_t)cell_index(g, xw2, ya, zb) * 4u;
+        load27(src.f, boff2, f);
+        const Gauge4 n2 = ldg4(src, boff2);
+        node(n2.ux, n2.uy, n2.uz, n2.sq, f, wab, q == 4, yz2, mid2);
       }
     }
     slice(xs, yz, mid);
@@ -500,7 +480,7 @@ __device__ __forceinline__ bool split27_once(const NodeSrcF& src, float* __restr
     float as = 0.f, ac = 0.f, bxs = 0.f, bxc = 0.f, bys = 0.f, byc = 0.f, bzs = 0.f, bzc = 0.f;
     double cs = 0.0;
     auto add = [&](int ex, int ey, int ez, float fn) {
-      fout[(int64_t)kIdx27.v[ex + 1][ey + 1][ez + 1] * P + cell] = fn;
+      fout[split27_fidx(kIdx27.v[ex + 1][ey + 1][ez + 1], cell, P)] = fn;
       kahan_add(as, ac, fn);
       kahan_add(bxs, bxc, (float)ex * fn);
       kahan_add(bys, byc, (float)ey * fn);
@@ -609,7 +589,7 @@ __device__ __forceinline__ void split27_finish(float* __restrict__ fout, float* 
   const int64_t P = a.g.plane;
   const float rf = (float)st.rho, dt = (float)a.dtau;
   for (int i = 0; i < 27; i++) {
-    const int64_t o = (int64_t)i * P + cell;
+    const int64_t o = split27_fidx(i, cell, P);
     float f = fout[o];
     f = __builtin_fmaf(dt, rf * a.w[i] - f, f);
     fout[o] = f;

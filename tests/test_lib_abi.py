@@ -67,12 +67,13 @@ def test_invalid_params_are_rejected_without_touching_the_gpu():
 
 
 def test_split_grid_limit_is_checked_before_allocation():
-    """The split step addresses nine FP32 population planes through a 32-bit buffer offset."""
+    """The split step addresses two FP32 population quad planes (16 bytes per cell each) through
+    a 32-bit buffer offset: 520^3 cells are more than it can address."""
     L = lbmpond_amd.lib()
     p = pondmod.PondParams()
     L.pond_params_default(ctypes.byref(p), 1)
     p.precision = 1
-    p.nx = p.ny = p.nz = 500
+    p.nx = p.ny = p.nz = 520
     h = ctypes.c_void_p()
     assert L.pond_create(ctypes.byref(p), ctypes.byref(h)) == 1
     assert b"POND_SPLIT D3Q27 supports up to" in L.pond_last_error(None)
