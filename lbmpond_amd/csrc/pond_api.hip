@@ -33,14 +33,13 @@ int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 
 // ---------------------------------------------------------------------------------- kernels
 
-// Cell::set_base (lbpack.cu:15-21) from the moment planes into the population planes
-template <typename FT>
-__global__ void k_set_base(FT* __restrict__ f, const double* __restrict__ rho, int64_t n, int64_t plane, int qn,
-                           const StepArgs a) {
+// Cell::set_base (lbpack.cu:15-21) from the moment planes into the FP64 populations
+__global__ void k_set_base(double* __restrict__ f, const double* __restrict__ rho, int64_t n, int64_t plane,
+                           int lattice, int qn, const StepArgs a) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const double r = rho[c];
-  for (int i = 0; i < qn; i++) f[(int64_t)i * plane + c] = (FT)(a.w[i] * r);
+  for (int i = 0; i < qn; i++) f[f64_fidx(lattice, i, c, plane)] = a.w[i] * r;
 }
 
 // split layout from FP64 moment planes: f_i = w_i*rho in FP32, (rho,u) in FP32, T in FP64
@@ -309,7 +308,7 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
     return bail(POND_ERR_HIP, "pond_create: hipStreamCreate failed");
   ctx->split = p->precision == POND_SPLIT;
   if (!ctx->split) {
-    const size_t fbytes = (size_t)ctx->Qn * g.plane * sizeof(double);
+    const size_t fbytes = (size_t)f64_fcount(p->lattice, ctx->Qn, g.plane) * sizeof(double);
     const size_t mbytes = (size_t)5 * g.plane * sizeof(double);
     for (int b = 0; b < 2; b++) {
       if (hipMalloc(&ctx->f64[b].f, fbytes) != hipSuccess || hipMalloc(&ctx->f64[b].m, mbytes) != hipSuccess)
@@ -446,11 +445,12 @@ pond_status pond_set_state(pond_ctx* ctx, const double* rho, const double* ux, c
   } else {
     pond_status s = upload_moments(ctx, ctx->f64[0].m, rho, ux, uy, uz, T);
     if (s != POND_OK) return s;
-    hipLaunchKernelGGL(k_set_base<double>, dim3(blocks), dim3(256), 0, ctx->stream, ctx->f64[0].f, ctx->f64[0].m,
-                       g.n, g.plane, ctx->Qn, ctx->args);
+    hipLaunchKernelGGL(k_set_base, dim3(blocks), dim3(256), 0, ctx->stream, ctx->f64[0].f, ctx->f64[0].m, g.n,
+                       g.plane, ctx->p.lattice, ctx->Qn, ctx->args);
     POND_HIP(ctx, hipGetLastError());
     // both buffers hold the initial state, as fill() does (init.cu:104-105)
-    POND_HIP(ctx, hipMemcpyAsync(ctx->f64[1].f, ctx->f64[0].f, (size_t)ctx->Qn * g.plane * sizeof(double),
+    POND_HIP(ctx, hipMemcpyAsync(ctx->f64[1].f, ctx->f64[0].f,
+                                 (size_t)f64_fcount(ctx->p.lattice, ctx->Qn, g.plane) * sizeof(double),
                                  hipMemcpyDeviceToDevice, ctx->stream));
     POND_HIP(ctx, hipMemcpyAsync(ctx->f64[1].m, ctx->f64[0].m, (size_t)5 * g.plane * sizeof(double),
                                  hipMemcpyDeviceToDevice, ctx->stream));
@@ -490,12 +490,15 @@ pond_status pond_set_populations(pond_ctx* ctx, const double* f, const double* r
       POND_HIP(ctx, hipMemcpy(ctx->sp[b].T, T, (size_t)g.n * sizeof(double), hipMemcpyHostToDevice));
     }
   } else {
+    const int64_t fcount = f64_fcount(ctx->p.lattice, ctx->Qn, g.plane);
+    std::vector<double> fs((size_t)fcount, 0.0);
+    for (int i = 0; i < ctx->Qn; i++)
+      for (int64_t c = 0; c < g.n; c++) fs[f64_fidx(ctx->p.lattice, i, c, g.plane)] = f[(size_t)i * g.n + c];
     for (int b = 0; b < 2; b++) {
       pond_status s = upload_moments(ctx, ctx->f64[b].m, rho, ux, uy, uz, T);
       if (s != POND_OK) return s;
-      for (int i = 0; i < ctx->Qn; i++)
-        POND_HIP(ctx, hipMemcpyAsync(ctx->f64[b].f + (size_t)i * g.plane, f + (size_t)i * g.n,
-                                     (size_t)g.n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+      // a blocking copy: fs is released at the end of this block
+      POND_HIP(ctx, hipMemcpy(ctx->f64[b].f, fs.data(), (size_t)fcount * sizeof(double), hipMemcpyHostToDevice));
     }
   }
   POND_HIP(ctx, hipMemsetAsync(ctx->niter, 0, (size_t)g.plane * sizeof(int), ctx->stream));
@@ -621,9 +624,11 @@ pond_status pond_get_populations(pond_ctx* ctx, double* f) {
       for (int64_t c = 0; c < g.n; c++) f[(size_t)i * g.n + c] = fs[split_fidx(ctx->p.lattice, i, c, g.plane)];
     return POND_OK;
   }
+  const int64_t fcount = f64_fcount(ctx->p.lattice, ctx->Qn, g.plane);
+  std::vector<double> fs((size_t)fcount);
+  POND_HIP(ctx, hipMemcpy(fs.data(), ctx->f64[ctx->cur].f, (size_t)fcount * sizeof(double), hipMemcpyDeviceToHost));
   for (int i = 0; i < ctx->Qn; i++)
-    POND_HIP(ctx, hipMemcpy(f + (size_t)i * g.n, ctx->f64[ctx->cur].f + (size_t)i * g.plane,
-                            (size_t)g.n * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t c = 0; c < g.n; c++) f[(size_t)i * g.n + c] = fs[f64_fidx(ctx->p.lattice, i, c, g.plane)];
   return POND_OK;
 }
 

@@ -82,8 +82,10 @@ constexpr int kPlanesPerRsrc64 = 9;
 constexpr int kRawBufferWord3 = 0x00020000;
 
 struct NodeBuf {
-  __amdgpu_buffer_rsrc_t f[3], m, P, ok;
+  // populations: D1Q5 planes 9j .. 9j+8 in f[j]; D3Q27 pair planes 4j .. 4j+3 in f[j]
+  __amdgpu_buffer_rsrc_t f[4], m, P, ok;
   uint32_t pbytes;                      // bytes per FP64 plane
+  uint32_t qbytes;                      // bytes per D3Q27 pair plane
   const double* __restrict__ fptr;      // for the out-of-line checked transfer
   Grid g;
 };
@@ -92,11 +94,21 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc64(const void* base, uint6
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)(uint32_t)bytes, kRawBufferWord3);
 }
 
+template <int LAT>
 __device__ __forceinline__ NodeBuf make_nodebuf(const NodeSrc& s) {
   NodeBuf b;
   const uint64_t pb = (uint64_t)s.g.plane * 8u;
   b.pbytes = (uint32_t)pb;
-  for (int j = 0; j < 3; j++) b.f[j] = rsrc64(s.f + (int64_t)j * kPlanesPerRsrc64 * s.g.plane, kPlanesPerRsrc64 * pb);
+  b.qbytes = (uint32_t)(2 * pb);
+  if (LAT == kD3Q27) {
+    for (int j = 0; j < 4; j++) {
+      const int cnt = 14 - 4 * j < 4 ? 14 - 4 * j : 4;
+      b.f[j] = rsrc64(s.f + (int64_t)8 * j * s.g.plane, (uint64_t)cnt * 2 * pb);
+    }
+  } else {
+    for (int j = 0; j < 3; j++) b.f[j] = rsrc64(s.f + (int64_t)j * kPlanesPerRsrc64 * s.g.plane, kPlanesPerRsrc64 * pb);
+    b.f[3] = b.f[2];
+  }
   b.m = rsrc64(s.m, 5 * pb);
   b.P = rsrc64(s.P, pb);
   b.ok = rsrc64(s.ok, (uint64_t)s.g.plane);
@@ -126,6 +138,24 @@ __device__ __forceinline__ bool node_ok(const NodeBuf& s, uint32_t boff) {
   return __builtin_amdgcn_raw_buffer_load_b8(s.ok, (int)(boff >> 3), 0, 0) != 0;
 }
 __device__ __forceinline__ const double* node_fptr(const NodeBuf& s) { return s.fptr; }
+
+// the 27 populations of a D3Q27 node (pair-plane layout) at byte offset boff of an FP64 plane
+__device__ __forceinline__ void node_f27(const NodeSrc& s, uint32_t boff, double f[27]) {
+#pragma unroll
+  for (int j = 0; j < 14; j++) {
+    const double2 v = *(const double2*)((const char*)(s.f + (int64_t)2 * j * s.g.plane) + 2u * boff);
+    f[2 * j] = v.x;
+    if (2 * j + 1 < 27) f[2 * j + 1] = v.y;
+  }
+}
+__device__ __forceinline__ void node_f27(const NodeBuf& s, uint32_t boff, double f[27]) {
+#pragma unroll
+  for (int j = 0; j < 14; j++) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(s.f[j >> 2], (int)(2u * boff), (int)((uint32_t)(j & 3) * s.qbytes), 0);
+    f[2 * j] = __builtin_bit_cast(double, ((uint64_t)(uint32_t)v[1] << 32) | (uint32_t)v[0]);
+    if (2 * j + 1 < 27) f[2 * j + 1] = __builtin_bit_cast(double, ((uint64_t)(uint32_t)v[3] << 32) | (uint32_t)v[2]);
+  }
+}
 
 // basis values of the node at byte offset boff for target t (structs.cuh:530-538).
 // e_iold*sqrt(|T0|)*sign(T0) for e_iold = -1, 0, +1 is exactly {-P, 0*P, P}.
@@ -164,7 +194,9 @@ __device__ __noinline__ double transfer27_checked(const double* __restrict__ f, 
 #pragma unroll
   for (int iold = 0; iold < 27; iold++) {
     const int jx = kE27[iold][0] + 1, jy = kE27[iold][1] + 1, jz = kE27[iold][2] + 1;
-    const double fadd = p2[jx][jy] * gz[jz] * ldp(f, iold, plane, boff);
+    // pair-plane layout (f64_fidx): boff = 8 * cell
+    const double fo = *(const double*)((const char*)(f + (int64_t)2 * (iold >> 1) * plane) + 2u * boff + 8u * (iold & 1));
+    const double fadd = p2[jx][jy] * gz[jz] * fo;
     if (finite_d(fadd)) fnew += fadd;
   }
   return fnew;
@@ -181,11 +213,13 @@ __device__ __forceinline__ double transfer27(const SRC& s, uint32_t boff, const 
   for (int a = 0; a < 3; a++)
 #pragma unroll
     for (int b = 0; b < 3; b++) p2[a][b] = t.W * gx[a] * gy[b];
+  double f[27];
+  node_f27(s, boff, f);
   double fnew = 0;
 #pragma unroll
   for (int iold = 0; iold < 27; iold++) {
     const int jx = kE27[iold][0] + 1, jy = kE27[iold][1] + 1, jz = kE27[iold][2] + 1;
-    fnew += p2[jx][jy] * gz[jz] * node_f(s, iold, boff);
+    fnew += p2[jx][jy] * gz[jz] * f[iold];
   }
   return fnew;
 }
@@ -477,7 +511,7 @@ __device__ bool f64_iterate(const SRC& src, const NodeSrc& gsrc, double* __restr
       m1 += fn * vy;
       m2 += fn * vz;
       m3 += fn * (vx * vx + vy * vy + vz * vz);
-      const int64_t o = (int64_t)i * P + cell;
+      const int64_t o = f64_fidx(LAT, i, cell, P);
       const double fo = niter > 1 ? fout[o] : (zero ? a.w[i] * 1.0 : gsrc.f[o]);
       if (fabs(fn - fo) >= a.tol_abs + a.tol_rel * fabs(fn)) conv = false;
       fout[o] = fn;
@@ -504,7 +538,7 @@ __device__ __forceinline__ void f64_finish(double* __restrict__ fout, double* __
                                            const StepArgs& a) {
   const int64_t P = a.g.plane;
   for (int i = 0; i < Q; i++) {
-    const int64_t o = (int64_t)i * P + cell;
+    const int64_t o = f64_fidx(Q == 27 ? kD3Q27 : kD1Q5, i, cell, P);
     double f = fout[o];
     const double feq = st.rho0 * a.w[i];
     f += a.dtau * (feq - f);
@@ -530,7 +564,7 @@ __global__ __launch_bounds__(64) void k_step_f64(const NodeSrc gsrc, double* __r
                                                  const StepArgs a) {
   using Src = typename std::conditional<BUF, NodeBuf, NodeSrc>::type;
   Src src;
-  if constexpr (BUF) src = make_nodebuf(gsrc);
+  if constexpr (BUF) src = make_nodebuf<LAT>(gsrc);
   else src = gsrc;
   const Grid& g = a.g;
   int x, y, z;
@@ -569,7 +603,7 @@ __global__ __launch_bounds__(64) void k_step_f64_rest(const NodeSrc gsrc, double
                                                       Pending pend, const StepArgs a) {
   using Src = typename std::conditional<BUF, NodeBuf, NodeSrc>::type;
   Src src;
-  if constexpr (BUF) src = make_nodebuf(gsrc);
+  if constexpr (BUF) src = make_nodebuf<LAT>(gsrc);
   else src = gsrc;
   const Grid& g = a.g;
   const int count = *pend.count;
@@ -765,7 +799,8 @@ __global__ void k_node_prep(const double* __restrict__ f, const double* __restri
   const double T = m[4 * plane + c];
   P[c] = sqrt(fabs(T)) * (double)isign_d(T);
   bool good = true;
-  for (int i = 0; i < Q; i++) good = good && fabs(f[(int64_t)i * plane + c]) < kSafe;
+  for (int i = 0; i < Q; i++)
+    good = good && fabs(f[f64_fidx(Q == 27 ? kD3Q27 : Q == 5 ? kD1Q5 : kD3Q125, i, c, plane)]) < kSafe;
   ok[c] = good ? 1 : 0;
 }
 
@@ -792,8 +827,10 @@ hipError_t launch_step_f64(int lattice, int interp, const double* fin, const dou
   }
   const unsigned blocks = (unsigned)((int64_t)a.g.ntx * a.g.nty * a.g.ntz);
   const NodeSrc src{fin, min, Pin, okin, a.g};
-  // buffer-resource loads while nine FP64 planes fit a 32-bit offset
-  const bool buf = (uint64_t)a.g.plane * 8u * kPlanesPerRsrc64 <= 0xffffffffull && !a.no_buf;
+  // buffer-resource loads while nine FP64 planes (D1Q5) or four pair planes (D3Q27) fit a 32-bit
+  // offset
+  const bool buf = (uint64_t)a.g.plane * 8u * (lattice == kD3Q27 ? 8u : (uint64_t)kPlanesPerRsrc64) <= 0xffffffffull &&
+                   !a.no_buf;
   hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, st, pend.count);
   const int64_t w2 = (a.g.n + 63) / 64;
   const unsigned blocks2 = (unsigned)(w2 < 16384 ? w2 : 16384);
