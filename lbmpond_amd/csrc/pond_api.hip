@@ -315,10 +315,11 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
         return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of the state buffers failed");
       ctx->bytes += (int64_t)(fbytes + mbytes);
     }
-    if (hipMalloc(&ctx->nodeP, (size_t)g.plane * sizeof(double)) != hipSuccess ||
+    // D3Q27: 32-byte gauge records (ux, uy, uz, P); otherwise one P plane
+    if (hipMalloc(&ctx->nodeP, (size_t)(p->lattice == POND_D3Q27 ? 4 : 1) * g.plane * sizeof(double)) != hipSuccess ||
         hipMalloc(&ctx->nodeok, (size_t)g.plane) != hipSuccess)
       return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of the node planes failed");
-    ctx->bytes += (int64_t)g.plane * 9;
+    ctx->bytes += (int64_t)g.plane * ((p->lattice == POND_D3Q27 ? 32 : 8) + 1);
   } else {
     const size_t fbytes = (size_t)split_fcount(p->lattice, ctx->Qn, g.plane) * sizeof(float);
     const size_t mbytes = (size_t)4 * g.plane * sizeof(float);

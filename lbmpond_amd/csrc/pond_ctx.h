@@ -13,14 +13,14 @@ namespace pond {
 
 // device buffers of one state (load or store) of the plain FP64 path
 struct F64Buf {
-  double* f = nullptr;  // Qn planes
+  double* f = nullptr;  // populations (f64_fidx layout)
   double* m = nullptr;  // rho, ux, uy, uz, T planes
 };
 
 // device buffers of one state of the split path: FP32 populations and (rho, ux, uy, uz),
 // FP64 temperature
 struct SplitBuf {
-  float* f = nullptr;   // Qn planes
+  float* f = nullptr;   // populations (split_fidx layout)
   float* m = nullptr;   // rho, ux, uy, uz planes
   double* T = nullptr;  // T plane
 };
@@ -77,7 +77,8 @@ struct pond_ctx {
   pond::F64Buf f64[2];
   pond::SplitBuf sp[2];
   pond::SplitArgs sargs{};
-  double* nodeP = nullptr;  // plain path: sqrt(|T|)*sign(T) per cell of the load buffer
+  double* nodeP = nullptr;  // plain path: per cell of the load buffer, D3Q27 the record (ux, uy, uz, P),
+                            // else P, with P = sqrt(|T|)*sign(T)
   unsigned char* nodeok = nullptr;  // plain path: populations finite and below 2^200
   float* sq = nullptr;      // split path: node gauge records (ux, uy, uz, sqrt(|T|)*sign(T)) of the load buffer
   float* guess = nullptr;   // split path: learned starting gauge increments (4 planes)

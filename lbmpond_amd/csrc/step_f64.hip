@@ -110,7 +110,7 @@ __device__ __forceinline__ NodeBuf make_nodebuf(const NodeSrc& s) {
     b.f[3] = b.f[2];
   }
   b.m = rsrc64(s.m, 5 * pb);
-  b.P = rsrc64(s.P, pb);
+  b.P = rsrc64(s.P, LAT == kD3Q27 ? 4 * pb : pb);
   b.ok = rsrc64(s.ok, (uint64_t)s.g.plane);
   b.fptr = s.f;
   b.g = s.g;
@@ -139,6 +139,27 @@ __device__ __forceinline__ bool node_ok(const NodeBuf& s, uint32_t boff) {
 }
 __device__ __forceinline__ const double* node_fptr(const NodeBuf& s) { return s.fptr; }
 
+// D3Q27: the node's gauge record (ux, uy, uz, sqrt(|T|)*sign(T)), 32 bytes per cell, written by
+// k_node_prep into the P buffer, at byte offset boff of an FP64 plane (boff = 8 * cell)
+__device__ __forceinline__ void node_g4(const NodeSrc& s, uint32_t boff, double& ux, double& uy, double& uz,
+                                       double& P) {
+  const double2 a = *(const double2*)((const char*)s.P + 4u * boff);
+  const double2 b = *(const double2*)((const char*)s.P + 4u * boff + 16u);
+  ux = a.x; uy = a.y; uz = b.x; P = b.y;
+}
+__device__ __forceinline__ double dbl_of(uint32_t lo, uint32_t hi) {
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ void node_g4(const NodeBuf& s, uint32_t boff, double& ux, double& uy, double& uz,
+                                       double& P) {
+  const auto a = __builtin_amdgcn_raw_buffer_load_b128(s.P, (int)(4u * boff), 0, 0);
+  const auto b = __builtin_amdgcn_raw_buffer_load_b128(s.P, (int)(4u * boff), 16, 0);
+  ux = dbl_of((uint32_t)a[0], (uint32_t)a[1]);
+  uy = dbl_of((uint32_t)a[2], (uint32_t)a[3]);
+  uz = dbl_of((uint32_t)b[0], (uint32_t)b[1]);
+  P = dbl_of((uint32_t)b[2], (uint32_t)b[3]);
+}
+
 // the 27 populations of a D3Q27 node (pair-plane layout) at byte offset boff of an FP64 plane
 __device__ __forceinline__ void node_f27(const NodeSrc& s, uint32_t boff, double f[27]) {
 #pragma unroll
@@ -162,8 +183,8 @@ __device__ __forceinline__ void node_f27(const NodeBuf& s, uint32_t boff, double
 template <class SRC>
 __device__ __forceinline__ bool basis27(const SRC& s, uint32_t boff, const Tgt27& t, double sqTb, double gx[3],
                                         double gy[3], double gz[3]) {
-  const double u0x = node_m(s, 1, boff), u0y = node_m(s, 2, boff), u0z = node_m(s, 3, boff);
-  const double Pn = node_P(s, boff);
+  double u0x, u0y, u0z, Pn;
+  node_g4(s, boff, u0x, u0y, u0z, Pn);
   const bool nok = node_ok(s, boff);
   const double Dx = (t.u1x - u0x) * sqTb;
   const double Dy = (t.u1y - u0y) * sqTb;
@@ -797,7 +818,14 @@ __global__ void k_node_prep(const double* __restrict__ f, const double* __restri
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const double T = m[4 * plane + c];
-  P[c] = sqrt(fabs(T)) * (double)isign_d(T);
+  const double Pc = sqrt(fabs(T)) * (double)isign_d(T);
+  if (Q == 27) {
+    // gauge record (ux, uy, uz, P) of the cell, 32 bytes
+    reinterpret_cast<double2*>(P)[2 * c] = make_double2(m[plane + c], m[2 * plane + c]);
+    reinterpret_cast<double2*>(P)[2 * c + 1] = make_double2(m[3 * plane + c], Pc);
+  } else {
+    P[c] = Pc;
+  }
   bool good = true;
   for (int i = 0; i < Q; i++)
     good = good && fabs(f[f64_fidx(Q == 27 ? kD3Q27 : Q == 5 ? kD1Q5 : kD3Q125, i, c, plane)]) < kSafe;
