@@ -295,6 +295,11 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
   a.g = g;
   a.no_buf = 0;
   if (const char* v = std::getenv("POND_NO_BUF")) a.no_buf = std::atoi(v) != 0;
+  // lane refill in the first pass of the FP64 D3Q27 step (POND_REFILL_CHUNK_F64 overrides, 0 =
+  // one tile per wave).  384^3 shock-bubble FP64 step: one tile per wave 8.07 s, chunks of 2
+  // tiles 6.38 s, 4 tiles 7.05 s (4 tiles in a single pass 6.90 s).
+  a.refill_chunk = 2;
+  if (const char* v = std::getenv("POND_REFILL_CHUNK_F64")) a.refill_chunk = std::atoi(v) > 0 ? std::atoi(v) : 0;
   // Iterations of the first pass of the two-pass step; cells still unconverged are finished by
   // the second pass (POND_PASS1_ITERS overrides).  Measured on the shock-bubble: FP64 D3Q27,
   // 128^3: K=3 2.98, K=5 3.19, K=8 3.35, single pass 2.86 MLUPS; split D3Q27, 192^3: K=3 7.6,

@@ -35,6 +35,7 @@ struct StepArgs {
   double inv_dim;     // 1./Dim (structs.cuh:330)
   int pass1_iters;    // iterations of the first pass of the one-lane-per-cell step
   int no_buf;         // 1: global loads even where the buffer-resource loads apply
+  int refill_chunk;   // D3Q27 first pass: 64-cell tiles per wave with lane refill (0: one tile per wave)
   double w[kQmax];    // lattice weights for the collide and set_base
 };
 

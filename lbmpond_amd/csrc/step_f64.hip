@@ -503,15 +503,14 @@ struct F64Iter {
   int niter;
 };
 
-// Fixed-point iterations (LBMpull, steps.cu:421-512) from state st until convergence (true) or
-// until the count reaches stop_at (false).
+// One fixed-point iteration (LBMpull, steps.cu:421-512) of one cell: true when it has converged.
 template <int LAT, int INTERP, class SRC>
-__device__ bool f64_iterate(const SRC& src, const NodeSrc& gsrc, double* __restrict__ fout, int64_t cell, int x, int y,
-                            int z, bool zero, F64Iter& st, int stop_at, const StepArgs& a) {
+__device__ __forceinline__ bool f64_once(const SRC& src, const NodeSrc& gsrc, double* __restrict__ fout, int64_t cell,
+                                         int x, int y, int z, bool zero, F64Iter& st, const StepArgs& a) {
   constexpr int Q = Lat<LAT>::Q;
   const int64_t P = a.g.plane;
   double* c0 = st.c0;
-  while (st.niter < stop_at) {
+  {
     st.niter++;
     const int niter = st.niter;
     const double uT[4] = {c0[0], c0[1], c0[2], c0[3]};
@@ -547,9 +546,31 @@ __device__ bool f64_iterate(const SRC& src, const NodeSrc& gsrc, double* __restr
       if (fabs(mn[k] - c0[k]) >= a.tol_abs + a.tol_rel * fabs(mn[k])) conv = false;
     st.rho0 = rho;
     c0[0] = m0; c0[1] = m1; c0[2] = m2; c0[3] = m3;
-    if (conv) return true;
+    return conv;
   }
+}
+
+// Fixed-point iterations from state st until convergence (true) or until the count reaches
+// stop_at (false).
+template <int LAT, int INTERP, class SRC>
+__device__ bool f64_iterate(const SRC& src, const NodeSrc& gsrc, double* __restrict__ fout, int64_t cell, int x, int y,
+                            int z, bool zero, F64Iter& st, int stop_at, const StepArgs& a) {
+  while (st.niter < stop_at)
+    if (f64_once<LAT, INTERP>(src, gsrc, fout, cell, x, y, z, zero, st, a)) return true;
   return false;
+}
+
+// starting state of a cell: its previous gauge; an all-zero cell is reset to set_base(1,
+// (0,0,0,TLat)) (steps.cu:360-363)
+__device__ __forceinline__ bool f64_start(const NodeSrc& gsrc, int64_t cell, const StepArgs& a, F64Iter& st) {
+  const int64_t P = a.g.plane;
+  st.rho0 = gsrc.m[cell];
+  st.c0[0] = gsrc.m[P + cell]; st.c0[1] = gsrc.m[2 * P + cell]; st.c0[2] = gsrc.m[3 * P + cell];
+  st.c0[3] = gsrc.m[4 * P + cell];
+  const bool zero = st.rho0 == 0 && st.c0[0] == 0 && st.c0[1] == 0 && st.c0[2] == 0 && st.c0[3] == 0;
+  if (zero) { st.rho0 = 1; st.c0[3] = a.tlat; }
+  st.niter = 0;
+  return zero;
 }
 
 // BGK towards rho*w_i in the converged gauge (steps.cu:557-564) and the cell's outputs
@@ -594,14 +615,7 @@ __global__ __launch_bounds__(64) void k_step_f64(const NodeSrc gsrc, double* __r
   bool pending = false;
   F64Iter st;
   if (active) {
-    const int64_t P = g.plane;
-    // c0 = load; all-zero cell -> set_base(1, (0,0,0,TLat)) (steps.cu:360-363)
-    st.rho0 = gsrc.m[cell];
-    st.c0[0] = gsrc.m[P + cell]; st.c0[1] = gsrc.m[2 * P + cell]; st.c0[2] = gsrc.m[3 * P + cell];
-    st.c0[3] = gsrc.m[4 * P + cell];
-    const bool zero = st.rho0 == 0 && st.c0[0] == 0 && st.c0[1] == 0 && st.c0[2] == 0 && st.c0[3] == 0;
-    if (zero) { st.rho0 = 1; st.c0[3] = a.tlat; }
-    st.niter = 0;
+    const bool zero = f64_start(gsrc, cell, a, st);
     const int stop = a.pass1_iters < a.max_iter ? a.pass1_iters : a.max_iter;
     const bool conv = f64_iterate<LAT, INTERP>(src, gsrc, fout, cell, x, y, z, zero, st, stop, a);
     pending = !conv && st.niter < a.max_iter;
@@ -614,6 +628,67 @@ __global__ __launch_bounds__(64) void k_step_f64(const NodeSrc gsrc, double* __r
     for (int k = 0; k < 4; k++) pend.g[k * pend.cap + e] = st.c0[k];
     pend.res[e] = st.rho0;
     pend.meta[e] = st.niter;
+  }
+}
+
+// First pass with lane refill: a wave owns `chunk` consecutive 64-cell tiles and its lanes take
+// the next unstarted cell of them whenever theirs converges or is set aside (after
+// a.pass1_iters iterations, to the pending list), instead of idling until the slowest cell of
+// their tile is done.  Each cell's arithmetic is that of k_step_f64.
+template <int LAT, int INTERP, bool BUF>
+__global__ __launch_bounds__(64) void k_step_f64_refill(const NodeSrc gsrc, double* __restrict__ fout,
+                                                        double* __restrict__ mout, int* __restrict__ niter_out,
+                                                        Pending pend, const StepArgs a, int chunk) {
+  using Src = typename std::conditional<BUF, NodeBuf, NodeSrc>::type;
+  Src src;
+  if constexpr (BUF) src = make_nodebuf<LAT>(gsrc);
+  else src = gsrc;
+  const Grid& g = a.g;
+  const int lane = threadIdx.x;
+  const int64_t ntiles = (int64_t)g.ntx * g.nty * g.ntz;
+  const int64_t t0 = xcd_block(blockIdx.x, g.xcd ? (int64_t)gridDim.x : -1) * chunk;
+  const int64_t total = (ntiles - t0 < chunk ? ntiles - t0 : chunk) * 64;  // cell slots of the chunk
+  const int stop = a.pass1_iters < a.max_iter ? a.pass1_iters : a.max_iter;
+  int64_t next = 0;  // first unstarted slot (the same in every lane)
+  bool active = false, zero = false;
+  int64_t cell = 0;
+  int x = 0, y = 0, z = 0;
+  F64Iter st;
+  while (true) {
+    const uint64_t want = __ballot(!active);
+    const int rank = __popcll(want & ((1ull << lane) - 1ull));
+    if (!active) {
+      const int64_t slot = next + rank;
+      if (slot < total && tile_cell(g, t0 + slot / 64, (int)(slot % 64), x, y, z)) {
+        cell = cell_index(g, x, y, z);
+        zero = f64_start(gsrc, cell, a, st);
+        active = true;
+      }
+    }
+    next += __popcll(want);
+    if (__ballot(active) == 0 && next >= total) break;
+    bool pending = false;
+    if (active) {
+      // the loop condition of f64_iterate, one iteration per trip
+      bool done = st.niter >= stop, conv = false;
+      if (!done) {
+        conv = f64_once<LAT, INTERP>(src, gsrc, fout, cell, x, y, z, zero, st, a);
+        done = conv || st.niter >= stop;
+      }
+      if (done) {
+        pending = !conv && st.niter < a.max_iter;
+        if (!pending) f64_finish<Lat<LAT>::Q>(fout, mout, niter_out, cell, st, a);
+        active = false;
+      }
+    }
+    bool any;
+    const int64_t e = pending_slot(pend, pending, any);
+    if (pending) {
+      pend.cell[e] = (int)cell;
+      for (int k = 0; k < 4; k++) pend.g[k * pend.cap + e] = st.c0[k];
+      pend.res[e] = st.rho0;
+      pend.meta[e] = st.niter;
+    }
   }
 }
 
@@ -868,7 +943,19 @@ hipError_t launch_step_f64(int lattice, int interp, const double* fin, const dou
     hipLaunchKernelGGL((k_step_f64_rest<L, I, B>), dim3(blocks2), dim3(64), 0, st, src, fout, mout, niter, pend, \
                        a);                                                                                       \
   } while (0)
-  if (lattice == kD3Q27) {
+#define POND_LAUNCH_R(L, I)                                                                                      \
+  do {                                                                                                           \
+    const unsigned w = (unsigned)(((int64_t)blocks + a.refill_chunk - 1) / a.refill_chunk);                       \
+    hipLaunchKernelGGL((k_step_f64_refill<L, I, true>), dim3(w), dim3(64), 0, st, src, fout, mout, niter, pend, a, \
+                       a.refill_chunk);                                                                          \
+    hipLaunchKernelGGL((k_step_f64_rest<L, I, true>), dim3(blocks2), dim3(64), 0, st, src, fout, mout, niter,    \
+                       pend, a);                                                                                 \
+  } while (0)
+  if (lattice == kD3Q27 && buf && a.refill_chunk > 0) {
+    if (interp == 0) POND_LAUNCH_R(kD3Q27, 0);
+    else if (interp == 1) POND_LAUNCH_R(kD3Q27, 1);
+    else POND_LAUNCH_R(kD3Q27, 2);
+  } else if (lattice == kD3Q27) {
     if (buf) {
       if (interp == 0) POND_LAUNCH(kD3Q27, 0, true);
       else if (interp == 1) POND_LAUNCH(kD3Q27, 1, true);
@@ -884,6 +971,7 @@ hipError_t launch_step_f64(int lattice, int interp, const double* fin, const dou
     else POND_LAUNCH(kD1Q5, 0, false);
   }
 #undef POND_LAUNCH
+#undef POND_LAUNCH_R
   return hipGetLastError();
 }
 

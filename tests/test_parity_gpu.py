@@ -203,3 +203,26 @@ def test_two_pass_fp64_is_independent_of_first_pass_length(gpu, monkeypatch):
             got = p.fields()
         for key in ("rho", "ux", "uy", "uz", "T", "niter"):
             assert np.array_equal(got[key], ref[key]), (k, key)
+
+
+@pytest.mark.parametrize("interp", ["weno5", "lagr4", "weno5c"])
+def test_fp64_lane_refill_matches_oracle(gpu, monkeypatch, interp):
+    """The FP64 first pass with lane refill (a wave owns several 64-cell tiles and its lanes take
+    new cells as theirs finish), with and without a second pass, gives the oracle's result bit for
+    bit, on a grid whose last tile is ragged."""
+    shape = (37, 9, 7)
+    f = smooth_random_state(shape, seed=72, T0=0.01)
+    o = po.Oracle("D3Q27", shape, interp)
+    o.set_state(*f)
+    o.step(3)
+    ref = o.fields()
+    for chunk, k1 in (("1", "8"), ("3", "2"), ("8", "1000")):
+        monkeypatch.setenv("POND_REFILL_CHUNK_F64", chunk)
+        monkeypatch.setenv("POND_PASS1_ITERS", k1)
+        print(f"[refill f64] {interp} chunk {chunk} first pass {k1}", flush=True)
+        with Pond("D3Q27", shape, interp, "fp64") as p:
+            p.set_state(*f)
+            p.step(3)
+            got = p.fields()
+        for key in ("rho", "ux", "uy", "uz", "T", "niter"):
+            assert np.array_equal(got[key], ref[key]), (interp, chunk, k1, key)
