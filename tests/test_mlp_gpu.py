@@ -55,7 +55,7 @@ def learned_vs_plain(case, collect=12, cont=6):
         f = p.populations()
         st = p.fields()
         pred = p.mlp_predict()
-        used = float((np.abs(pred).max(axis=0) <= 0.1).mean())
+        used = float((np.abs(pred).max(axis=0) <= 0.3).mean())
         iters_mlp = []
         dep_mlp = CapDependence(case.shape, p.params.max_iter)
         for _ in range(cont):
