@@ -36,6 +36,7 @@ from lbmpond_amd.compare import CapDependence, state_errors  # noqa: E402
 METRIC = "MLUPS + mean gauge iters/step, learned+FP32 vs plain FP64, at stated L∞ err"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+VALU_ISSUE_PEAK = 256 * 4 * 16 * 2.4e9 / 1e12  # T lane-instructions/s
 
 # algorithmic bytes per lattice update: read one state, write one state, int32 iterations
 BYTES_PER_LUP = {("D3Q27", "fp64"): 2 * (27 * 8 + 5 * 8) + 4, ("D3Q27", "split"): 2 * (27 * 4 + 4 * 4 + 8) + 4,
@@ -154,21 +155,26 @@ def cpu_baseline(case_name, interp):
                       f"{cells * niter / dt / 1e6:.5f} M cell-iterations/s)"}
 
 
-def profiled_traffic(kernel, workload):
-    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3 counter
-    summaries: a line 'traffic: kernel=K workload=W bytes_per_launch=B' in profiles/*.md (FETCH_SIZE
-    and WRITE_SIZE per dispatch, scaled by the same counters' reading of a plain device copy)."""
+def profiled(tag, kernel, workload, key):
+    """A per-launch figure of `kernel` on `workload` from the committed rocprofv3 counter
+    summaries: a line 'TAG: kernel=K workload=W KEY=V' in profiles/*.md.  Returns (V, file)."""
     import glob
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         with open(path) as fh:
             for line in fh:
                 line = line.strip().strip("`")
-                if not line.startswith("traffic:"):
+                if not line.startswith(tag + ":"):
                     continue
-                kv = dict(t.split("=", 1) for t in line[len("traffic:"):].split() if "=" in t)
-                if kv.get("kernel") == kernel and kv.get("workload") == workload:
-                    return float(kv["bytes_per_launch"]), os.path.relpath(path, ROOT)
+                kv = dict(t.split("=", 1) for t in line[len(tag) + 1:].split() if "=" in t)
+                if kv.get("kernel") == kernel and kv.get("workload") == workload and key in kv:
+                    return float(kv[key]), os.path.relpath(path, ROOT)
     return None, None
+
+
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the same counters'
+    reading of a plain device copy)"""
+    return profiled("traffic", kernel, workload, "bytes_per_launch")
 
 
 def copy_calibration(nbytes):
@@ -256,7 +262,9 @@ def main():
     flops = FLOP_PER_CELL_ITER[case.lattice] * head["niter_mean"] * cells
     err = summary[head_name].get("rel_linf_vs_fp64")
     kernel = KERNEL_NAME[(case.lattice, prec)]
-    traffic, traffic_src = profiled_traffic(kernel, f"{case.name}-{case.lattice}-{'x'.join(map(str, case.shape))}")
+    workload = f"{case.name}-{case.lattice}-{'x'.join(map(str, case.shape))}"
+    traffic, traffic_src = profiled_traffic(kernel, workload)
+    valu_insts, valu_src = profiled("valu", kernel, workload, "valu_wave_insts_per_launch")
     out = {
         "metric": METRIC,
         "value": round(head["mlups"], 4),
@@ -284,8 +292,17 @@ def main():
         "valu_roofline": {"bound": "fp64-valu", "achieved": round(flops / launch_s / 1e12, 4), "peak": FP64_PEAK_TFLOPS,
                           "unit": "TFLOP/s", "frac": round(flops / launch_s / 1e12 / FP64_PEAK_TFLOPS, 5),
                           "flop_per_cell_iter": FLOP_PER_CELL_ITER[case.lattice],
-                          "note": "reference-algorithm FP64 flops at the path's mean iterations"},
+                          "note": "the reference algorithm's FP64 flops (19 per transfer term, de-duplicated "
+                                  "transfers) at the path's mean iterations; the split path computes the same "
+                                  "result with sum-factorised FP32 transfers, so its figure can exceed the peak"},
     }
+    if valu_insts is not None:
+        # executed VALU instructions of the kernel (rocprofv3 SQ_INSTS_VALU per launch) over this
+        # run's launch time, against the issue rate: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+        rate = valu_insts * 64 / launch_s / 1e12
+        out["valu_issue"] = {"bound": "valu-issue", "achieved": round(rate, 3), "peak": VALU_ISSUE_PEAK,
+                             "unit": "T lane-instructions/s", "frac": round(rate / VALU_ISSUE_PEAK, 4),
+                             "kernel": kernel, "source": valu_src}
     if not args.no_cpu:
         log("[bench] cpu baseline")
         out["cpu_baseline"] = cpu_baseline(args.config, interp_for(case))
