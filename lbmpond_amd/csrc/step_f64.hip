@@ -86,7 +86,6 @@ struct NodeBuf {
   __amdgpu_buffer_rsrc_t f[4], m, P, ok;
   uint32_t pbytes;                      // bytes per FP64 plane
   uint32_t qbytes;                      // bytes per D3Q27 pair plane
-  const double* __restrict__ fptr;      // for the out-of-line checked transfer
   Grid g;
 };
 
@@ -112,7 +111,6 @@ __device__ __forceinline__ NodeBuf make_nodebuf(const NodeSrc& s) {
   b.m = rsrc64(s.m, 5 * pb);
   b.P = rsrc64(s.P, LAT == kD3Q27 ? 4 * pb : pb);
   b.ok = rsrc64(s.ok, (uint64_t)s.g.plane);
-  b.fptr = s.f;
   b.g = s.g;
   return b;
 }
@@ -122,7 +120,6 @@ __device__ __forceinline__ double node_f(const NodeSrc& s, int k, uint32_t boff)
 __device__ __forceinline__ double node_m(const NodeSrc& s, int k, uint32_t boff) { return ldp(s.m, k, s.g.plane, boff); }
 __device__ __forceinline__ double node_P(const NodeSrc& s, uint32_t boff) { return ldp(s.P, 0, s.g.plane, boff); }
 __device__ __forceinline__ bool node_ok(const NodeSrc& s, uint32_t boff) { return s.ok[boff >> 3] != 0; }
-__device__ __forceinline__ const double* node_fptr(const NodeSrc& s) { return s.f; }
 
 __device__ __forceinline__ double ldb64(__amdgpu_buffer_rsrc_t r, uint32_t boff, uint32_t soff) {
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)boff, (int)soff, 0));
@@ -137,7 +134,6 @@ __device__ __forceinline__ double node_P(const NodeBuf& s, uint32_t boff) { retu
 __device__ __forceinline__ bool node_ok(const NodeBuf& s, uint32_t boff) {
   return __builtin_amdgcn_raw_buffer_load_b8(s.ok, (int)(boff >> 3), 0, 0) != 0;
 }
-__device__ __forceinline__ const double* node_fptr(const NodeBuf& s) { return s.fptr; }
 
 // D3Q27: the node's gauge record (ux, uy, uz, sqrt(|T|)*sign(T)), 32 bytes per cell, written by
 // k_node_prep into the P buffer, at byte offset boff of an FP64 plane (boff = 8 * cell)
@@ -202,33 +198,11 @@ __device__ __forceinline__ bool basis27(const SRC& s, uint32_t boff, const Tgt27
   return safe;
 }
 
-// the reference's term loop with NaN/Inf terms dropped (structs.cuh:539-542); kept out of line,
-// it runs only for nodes that fail the magnitude test
-__device__ __noinline__ double transfer27_checked(const double* __restrict__ f, int64_t plane, uint32_t boff,
-                                                  double W, const double* gx, const double* gy, const double* gz) {
-  double p2[3][3];
-#pragma unroll
-  for (int a = 0; a < 3; a++)
-#pragma unroll
-    for (int b = 0; b < 3; b++) p2[a][b] = W * gx[a] * gy[b];
-  double fnew = 0;
-#pragma unroll
-  for (int iold = 0; iold < 27; iold++) {
-    const int jx = kE27[iold][0] + 1, jy = kE27[iold][1] + 1, jz = kE27[iold][2] + 1;
-    // pair-plane layout (f64_fidx): boff = 8 * cell
-    const double fo = *(const double*)((const char*)(f + (int64_t)2 * (iold >> 1) * plane) + 2u * boff + 8u * (iold & 1));
-    const double fadd = p2[jx][jy] * gz[jz] * fo;
-    if (finite_d(fadd)) fnew += fadd;
-  }
-  return fnew;
-}
-
 // gauge transfer of the node at byte offset boff to target t (structs.cuh:525-548)
 template <class SRC>
 __device__ __forceinline__ double transfer27(const SRC& s, uint32_t boff, const Tgt27& t, double sqTb) {
   double gx[3], gy[3], gz[3];
   const bool safe = basis27(s, boff, t, sqTb, gx, gy, gz);
-  if (!safe) return transfer27_checked(node_fptr(s), s.g.plane, boff, t.W, gx, gy, gz);
   double p2[3][3];
 #pragma unroll
   for (int a = 0; a < 3; a++)
@@ -241,6 +215,17 @@ __device__ __forceinline__ double transfer27(const SRC& s, uint32_t boff, const 
   for (int iold = 0; iold < 27; iold++) {
     const int jx = kE27[iold][0] + 1, jy = kE27[iold][1] + 1, jz = kE27[iold][2] + 1;
     fnew += p2[jx][jy] * gz[jz] * f[iold];
+  }
+  if (!safe) {
+    // the reference's term loop with NaN/Inf terms dropped (structs.cuh:539-542), for the
+    // nodes that fail the magnitude test; in line, so the basis values stay in registers
+    fnew = 0;
+#pragma unroll
+    for (int iold = 0; iold < 27; iold++) {
+      const int jx = kE27[iold][0] + 1, jy = kE27[iold][1] + 1, jz = kE27[iold][2] + 1;
+      const double fadd = p2[jx][jy] * gz[jz] * f[iold];
+      if (finite_d(fadd)) fnew += fadd;
+    }
   }
   return fnew;
 }
