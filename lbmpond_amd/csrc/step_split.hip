@@ -697,10 +697,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
 // arithmetic is the same as in k_step_split27 (bitwise the same results); the active cells of
 // a wave stay within a window of the chunk, so node loads remain close together.
 template <int INTERP>
-__global__ __launch_bounds__(64) void k_step_split27_refill(
-    const float* __restrict__ fin, const float* __restrict__ min, const double* __restrict__ Tin,
-    const float* __restrict__ g4in, float* __restrict__ fout, float* __restrict__ mout, double* __restrict__ Tout,
-    int* __restrict__ niter_out, const float* __restrict__ guess, const SplitArgs a, int chunk) {
+__device__ __forceinline__ void split27_refill(const float* __restrict__ fin, const float* __restrict__ min,
+                                               const double* __restrict__ Tin, const float* __restrict__ g4in,
+                                               float* __restrict__ fout, float* __restrict__ mout,
+                                               double* __restrict__ Tout, int* __restrict__ niter_out,
+                                               const float* __restrict__ guess, const SplitArgs& a, int chunk) {
   const Grid& g = a.g;
   const int lane = threadIdx.x;
   const int64_t ntiles = (int64_t)g.ntx * g.nty * g.ntz;
@@ -739,6 +740,14 @@ __global__ __launch_bounds__(64) void k_step_split27_refill(
       }
     }
   }
+}
+
+template <int INTERP>
+__global__ __launch_bounds__(64) void k_step_split27_refill(
+    const float* __restrict__ fin, const float* __restrict__ min, const double* __restrict__ Tin,
+    const float* __restrict__ g4in, float* __restrict__ fout, float* __restrict__ mout, double* __restrict__ Tout,
+    int* __restrict__ niter_out, const float* __restrict__ guess, const SplitArgs a, int chunk) {
+  split27_refill<INTERP>(fin, min, Tin, g4in, fout, mout, Tout, niter_out, guess, a, chunk);
 }
 
 // Second pass: a fixed grid of waves walks the pending list; every cell resumes its iteration
@@ -1012,6 +1021,8 @@ hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const
                        out.m, out.T, niter, pend, a);                                                             \
   } while (0)
   if (interp == 2 && a.weno5c_variant == 0 && a.refill_chunk > 0 && a.pass1_iters >= a.max_iter + kLearnedBudget) {
+    // (measured alternatives at 384^3: one x slice per trip 372 ms, held to 3 waves per SIMD
+    // with spills 360-400 ms, against 317 ms for two slices at 2 waves per SIMD)
     const int64_t waves = ((int64_t)blocks + a.refill_chunk - 1) / a.refill_chunk;
     hipLaunchKernelGGL(k_step_split27_refill<2>, dim3((unsigned)waves), dim3(64), 0, st, in.f, in.m, in.T, g4, out.f,
                        out.m, out.T, niter, guess, a, a.refill_chunk);
