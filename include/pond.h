@@ -137,6 +137,14 @@ pond_status pond_timing_info(pond_ctx* ctx, pond_timing* out);
 /* write Rho/Vx/Vy/Vz/T_%07d.arr of the current state into directory dir (drop.cu:2-34) */
 pond_status pond_drop(pond_ctx* ctx, const char* dir);
 
+/* checkpoint of the current state: populations, moments, per-cell iteration counts and the step
+   count, in the device layout, so that pond_checkpoint_read into a context with the same
+   lattice, precision and grid restarts bit for bit.  No counterpart in the reference, whose
+   drop() (drop.cu:2-34) writes the moments only.  The learned initializer's network is not
+   part of it (pond_mlp_get_params / pond_mlp_set_params). */
+pond_status pond_checkpoint_write(pond_ctx* ctx, const char* path);
+pond_status pond_checkpoint_read(pond_ctx* ctx, const char* path);
+
 /* read one .arr file written by pond_drop (or by the reference's drop()) into out
    (nx*ny*nz doubles); dims[3] receives Nx,Ny,Nz.  out may be NULL to query dims. */
 pond_status pond_read_arr(const char* path, int64_t* dims, double* out);

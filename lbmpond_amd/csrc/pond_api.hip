@@ -710,6 +710,112 @@ pond_status pond_drop(pond_ctx* ctx, const char* dir) {
   return POND_OK;
 }
 
+// ---- checkpoint: the full state in the device layout, for an exact restart
+namespace {
+struct CkptHeader {
+  char magic[8];  // "PONDCKPT"
+  int32_t version, lattice, precision, interp;
+  int64_t nx, ny, nz, plane, step;
+  int64_t fbytes, mbytes, tbytes, nbytes;  // section sizes: populations, moments, T, niter
+};
+
+struct CkptSections {
+  void* f; void* m; void* T;
+  int64_t fbytes, mbytes, tbytes, nbytes;
+};
+
+CkptSections ckpt_sections(pond_ctx* ctx, int b) {
+  const Grid& g = ctx->g;
+  CkptSections c{};
+  if (ctx->split) {
+    c.f = ctx->sp[b].f; c.m = ctx->sp[b].m; c.T = ctx->sp[b].T;
+    c.fbytes = split_fcount(ctx->p.lattice, ctx->Qn, g.plane) * (int64_t)sizeof(float);
+    c.mbytes = 4 * g.plane * (int64_t)sizeof(float);
+    c.tbytes = g.plane * (int64_t)sizeof(double);
+  } else {
+    c.f = ctx->f64[b].f; c.m = ctx->f64[b].m; c.T = nullptr;
+    c.fbytes = f64_fcount(ctx->p.lattice, ctx->Qn, g.plane) * (int64_t)sizeof(double);
+    c.mbytes = 5 * g.plane * (int64_t)sizeof(double);
+    c.tbytes = 0;
+  }
+  c.nbytes = g.plane * (int64_t)sizeof(int);
+  return c;
+}
+}  // namespace
+
+pond_status pond_checkpoint_write(pond_ctx* ctx, const char* path) {
+  if (!ctx || !path) return POND_ERR_INVALID;
+  if (!ctx->has_state) return fail(ctx, POND_ERR_STATE, "pond_checkpoint_write: no state set");
+  pond_status s = pond_sync(ctx);
+  if (s != POND_OK) return s;
+  const Grid& g = ctx->g;
+  const CkptSections c = ckpt_sections(ctx, ctx->cur);
+  CkptHeader h{};
+  std::memcpy(h.magic, "PONDCKPT", 8);
+  h.version = 1; h.lattice = ctx->p.lattice; h.precision = ctx->p.precision; h.interp = ctx->p.interp;
+  h.nx = g.nx; h.ny = g.ny; h.nz = g.nz; h.plane = g.plane; h.step = ctx->step;
+  h.fbytes = c.fbytes; h.mbytes = c.mbytes; h.tbytes = c.tbytes; h.nbytes = c.nbytes;
+  FILE* fp = std::fopen(path, "wb");
+  if (!fp) return fail(ctx, POND_ERR_IO, std::string("pond_checkpoint_write: cannot open ") + path);
+  bool ok = std::fwrite(&h, sizeof h, 1, fp) == 1;
+  const void* src[4] = {c.f, c.m, c.T, ctx->niter};
+  const int64_t len[4] = {c.fbytes, c.mbytes, c.tbytes, c.nbytes};
+  std::vector<char> buf;
+  for (int k = 0; k < 4 && ok; k++) {
+    if (len[k] == 0) continue;
+    buf.resize((size_t)len[k]);
+    if (hipMemcpy(buf.data(), src[k], (size_t)len[k], hipMemcpyDeviceToHost) != hipSuccess) {
+      std::fclose(fp);
+      return fail(ctx, POND_ERR_HIP, "pond_checkpoint_write: device copy failed");
+    }
+    ok = std::fwrite(buf.data(), 1, (size_t)len[k], fp) == (size_t)len[k];
+  }
+  ok = (std::fclose(fp) == 0) && ok;
+  if (!ok) return fail(ctx, POND_ERR_IO, std::string("pond_checkpoint_write: write failed: ") + path);
+  return POND_OK;
+}
+
+pond_status pond_checkpoint_read(pond_ctx* ctx, const char* path) {
+  if (!ctx || !path) return POND_ERR_INVALID;
+  POND_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  FILE* fp = std::fopen(path, "rb");
+  if (!fp) return fail(ctx, POND_ERR_IO, std::string("pond_checkpoint_read: cannot open ") + path);
+  const Grid& g = ctx->g;
+  CkptHeader h{};
+  const CkptSections c = ckpt_sections(ctx, 0);
+  bool ok = std::fread(&h, sizeof h, 1, fp) == 1;
+  if (!ok || std::memcmp(h.magic, "PONDCKPT", 8) != 0 || h.version != 1) {
+    std::fclose(fp);
+    return fail(ctx, POND_ERR_IO, std::string("pond_checkpoint_read: not a checkpoint: ") + path);
+  }
+  if (h.lattice != ctx->p.lattice || h.precision != ctx->p.precision || h.nx != g.nx || h.ny != g.ny ||
+      h.nz != g.nz || h.plane != g.plane || h.fbytes != c.fbytes || h.mbytes != c.mbytes || h.tbytes != c.tbytes ||
+      h.nbytes != c.nbytes) {
+    std::fclose(fp);
+    return fail(ctx, POND_ERR_INVALID, "pond_checkpoint_read: lattice, precision or grid differ from the context");
+  }
+  const int64_t len[4] = {c.fbytes, c.mbytes, c.tbytes, c.nbytes};
+  std::vector<char> sec[4];
+  for (int k = 0; k < 4 && ok; k++) {
+    sec[k].resize((size_t)len[k]);
+    if (len[k]) ok = std::fread(sec[k].data(), 1, (size_t)len[k], fp) == (size_t)len[k];
+  }
+  std::fclose(fp);
+  if (!ok) return fail(ctx, POND_ERR_IO, std::string("pond_checkpoint_read: truncated file: ") + path);
+  // both buffers hold the state, as after pond_set_state
+  for (int b = 0; b < 2; b++) {
+    const CkptSections d = ckpt_sections(ctx, b);
+    void* dst[3] = {d.f, d.m, d.T};
+    for (int k = 0; k < 3; k++)
+      if (len[k]) POND_HIP(ctx, hipMemcpy(dst[k], sec[k].data(), (size_t)len[k], hipMemcpyHostToDevice));
+  }
+  POND_HIP(ctx, hipMemcpy(ctx->niter, sec[3].data(), (size_t)len[3], hipMemcpyHostToDevice));
+  ctx->cur = 0;
+  ctx->step = h.step;
+  ctx->has_state = true;
+  return POND_OK;
+}
+
 pond_status pond_read_arr(const char* path, int64_t* dims, double* out) {
   if (!path) return POND_ERR_INVALID;
   FILE* fp = std::fopen(path, "rb");

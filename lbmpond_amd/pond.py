@@ -102,6 +102,8 @@ SIGNATURES = {
     "pond_diagnostics": (ctypes.c_int, [_CTX, ctypes.POINTER(PondDiag)]),
     "pond_timing_info": (ctypes.c_int, [_CTX, ctypes.POINTER(PondTiming)]),
     "pond_drop": (ctypes.c_int, [_CTX, ctypes.c_char_p]),
+    "pond_checkpoint_write": (ctypes.c_int, [_CTX, ctypes.c_char_p]),
+    "pond_checkpoint_read": (ctypes.c_int, [_CTX, ctypes.c_char_p]),
     "pond_read_arr": (ctypes.c_int, [ctypes.c_char_p, _I64, _D]),
     "pond_device_bytes": (ctypes.c_int64, [_CTX]),
     "pond_mlp_num_params": (ctypes.c_int64, []),
@@ -282,6 +284,14 @@ class Pond:
 
     def drop(self, directory):
         self._check(lib().pond_drop(self._h, str(directory).encode()), "pond_drop")
+
+    def checkpoint(self, path):
+        """the full state (populations included) for an exact restart (pond_checkpoint_write)"""
+        self._check(lib().pond_checkpoint_write(self._h, str(path).encode()), "pond_checkpoint_write")
+
+    def restore(self, path):
+        """state and step count from a checkpoint of the same lattice, precision and grid"""
+        self._check(lib().pond_checkpoint_read(self._h, str(path).encode()), "pond_checkpoint_read")
 
     def device_bytes(self):
         return int(lib().pond_device_bytes(self._h))

@@ -226,3 +226,31 @@ def test_fp64_lane_refill_matches_oracle(gpu, monkeypatch, interp):
             got = p.fields()
         for key in ("rho", "ux", "uy", "uz", "T", "niter"):
             assert np.array_equal(got[key], ref[key]), (interp, chunk, k1, key)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_checkpoint_restart_is_exact(gpu, tmp_path, precision):
+    """A run restored from a checkpoint continues bit for bit like the uninterrupted run (state,
+    iteration counts and step count); a checkpoint of another grid is refused."""
+    shape = (20, 9, 7)
+    f = smooth_random_state(shape, seed=73, T0=0.01)
+    path = tmp_path / "state.ckpt"
+    with Pond("D3Q27", shape, "weno5c", precision) as p:
+        p.set_state(*f)
+        p.step(3)
+        p.checkpoint(path)
+        p.step(2)
+        ref = p.fields()
+        ref_pop = p.populations()
+        ref_step = p.diagnostics()["step"]
+    with Pond("D3Q27", shape, "weno5c", precision) as q:
+        q.restore(path)
+        q.step(2)
+        got = q.fields()
+        assert np.array_equal(q.populations(), ref_pop)
+        assert q.diagnostics()["step"] == ref_step
+    for k in ("rho", "ux", "uy", "uz", "T", "niter"):
+        assert np.array_equal(got[k], ref[k]), k
+    with Pond("D3Q27", (20, 9, 8), "weno5c", precision) as r:
+        with pytest.raises(Exception):
+            r.restore(path)
