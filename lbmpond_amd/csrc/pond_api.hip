@@ -288,8 +288,8 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
   // Row patches of the linear tile order (POND_ROW_PATCH=edge overrides; ignored where the grid
   // does not divide into them).  Measured on the shock-bubble D3Q27 WENO5C at 384^3: split step
   // 1124 ms in row order, 1025 ms with 4x4 patches, 1063-1075 ms with 8, 16 or 32; the FP64
-  // step 12.70 s in row order, 12.92 s with 8x8 patches.
-  int rp = p->precision == POND_SPLIT ? 4 : 0;
+  // step (pair planes, lane refill) 4.10 s in row order, 3.13 s with 4x4 patches, 3.14 s with 8x8.
+  int rp = 4;
   if (const char* v = std::getenv("POND_ROW_PATCH")) rp = std::atoi(v);
   g.rp = (rp > 0 && g.tx == 64 && g.nx % 64 == 0 && g.ny % rp == 0 && g.nz % rp == 0) ? rp : 0;
   a.g = g;
