@@ -680,14 +680,6 @@ __global__ __launch_bounds__(64) void k_step_split27(const float* __restrict__ f
   split27_pass1<INTERP>(fin, min, Tin, g4in, fout, mout, Tout, niter_out, guess, pend, a);
 }
 
-// the grouped WENO5C step held to 168 VGPRs (3 waves per SIMD; a few spilled registers)
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_step_split27_g3(
-    const float* __restrict__ fin, const float* __restrict__ min, const double* __restrict__ Tin,
-    const float* __restrict__ g4in, float* __restrict__ fout, float* __restrict__ mout, double* __restrict__ Tout,
-    int* __restrict__ niter_out, const float* __restrict__ guess, SplitPending pend, const SplitArgs a) {
-  split27_pass1<2>(fin, min, Tin, g4in, fout, mout, Tout, niter_out, guess, pend, a);
-}
-
 // Single pass with lane refill.  A wave owns a chunk of `chunk` consecutive 64-cell tiles.  Its
 // lanes start on the first tile; whenever lanes finish their cells, they take the chunk's next
 // unstarted cells in order (one ballot per iteration), so a lane is idle only at the end of the
@@ -1029,7 +1021,6 @@ hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const
     return hipGetLastError();
   }
   if (interp == 1) POND_LAUNCH_S(k_step_split27<1>, 1);
-  else if (interp == 2 && a.weno5c_variant == 1) POND_LAUNCH_S(k_step_split27_g3, 2);
   else if (interp == 2 && a.weno5c_variant == 2) POND_LAUNCH_S(k_step_split27<3>, 3);
   else if (interp == 2 && a.weno5c_variant == 3) POND_LAUNCH_S(k_step_split27<4>, 4);
   else if (interp == 2) POND_LAUNCH_S(k_step_split27<2>, 2);
