@@ -284,6 +284,11 @@ def main():
                    "paths": paths},
         "niter_mean": round(head["niter_mean"], 4),
         "rel_linf_vs_fp64": err,
+        # the same error outside the cells that a cell at the iteration cap (which ends its step
+        # on an unconverged iterate of a cycle, in the FP64 reference too) can have influenced
+        "rel_linf_vs_fp64_outside_cap_dependence": summary[head_name].get("rel_linf_vs_fp64_outside_cap_dependence"),
+        "cap_dependence_fraction": (round(summary[head_name]["cells_in_cap_dependence"] / cells, 5)
+                                    if "cells_in_cap_dependence" in summary[head_name] else None),
         "speedup_vs_fp64": round(head["mlups"] / ref["mlups"], 3) if ref is not None and head is not ref else None,
         "paths": summary,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
