@@ -32,6 +32,15 @@ constexpr int kE27[27][3] = {
     {0, -1, -1}, {1, -1, -1},
     {-1, -1, -1}};
 
+// population index of D3Q27 velocity (jx,jy,jz), components in {-1,0,1} mapped to 0..2
+struct Idx27 {
+  int v[3][3][3];
+  constexpr Idx27() : v() {
+    for (int i = 0; i < 27; i++) v[kE27[i][0] + 1][kE27[i][1] + 1][kE27[i][2] + 1] = i;
+  }
+};
+constexpr Idx27 kIdx27{};
+
 // per-axis abscissae of D1Q5 / D3Q125 in table order
 constexpr double kA5[5] = {0.0, kEc1, -kEc1, kEc2, -kEc2};
 

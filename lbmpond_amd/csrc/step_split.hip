@@ -27,15 +27,6 @@
 
 namespace pond {
 
-// population index of D3Q27 velocity (jx,jy,jz), components in {-1,0,1} mapped to 0..2
-struct Idx27 {
-  int v[3][3][3];
-  constexpr Idx27() : v() {
-    for (int i = 0; i < 27; i++) v[kE27[i][0] + 1][kE27[i][1] + 1][kE27[i][2] + 1] = i;
-  }
-};
-constexpr Idx27 kIdx27{};
-
 // per-lane constants of one target population in the current gauge
 struct TgtF {
   float u1x, u1y, u1z;     // target gauge velocity, rounded to FP32 ...
@@ -321,6 +312,19 @@ __device__ void interp_group_weno5c(const NodeSrcF& src, int x, int y, int z, do
     c[t] = (float)(cx - r[t]);
   }
   const int xlo = min(r[0], min(r[1], r[2])) - 2, xhi = max(r[0], max(r[1], r[2])) + 2;
+  // (64-bit difference: for an infinite temperature the windows sit at +-2^30 and the 32-bit
+  // difference would overflow)
+  if ((int64_t)xhi - (int64_t)xlo > 8) {
+    // windows far apart (a gauge with a huge temperature or velocity): target by target, so the
+    // loop stays five slices per target whatever the gauge
+#pragma unroll 1
+    for (int t = 0; t < 3; t++) {
+      const int i = kIdx27.v[t][ey + 1][ez + 1];
+      const double cx = (double)x - (s * (double)(t - 1) + gc[0]);
+      out[t] = interp_weno5f<true>(src, cx, cy, cz, make_tgtf(i, gc), sqTb);
+    }
+    return;
+  }
   float S[3][3], ub[3][5];
 #pragma unroll
   for (int t = 0; t < 3; t++) {
