@@ -169,8 +169,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       live = x < g.nx;
       cell = cell_index(g, live ? x : 0, y, z);
       const int ec = 4 * 66 + 16 * sub + col + 1;  // the cell itself: dx = dy = dz = 0
-      const float rci = 1.0f / lm[wv][ec][0], uxc = lm[wv][ec][1], uyc = lm[wv][ec][2], uzc = lm[wv][ec][3];
-      const float Tci = 1.0f / lm[wv][ec][4];
+      const float rc = lm[wv][ec][0], rci = 1.0f / rc, uxc = lm[wv][ec][1], uyc = lm[wv][ec][2], uzc = lm[wv][ec][3];
+      const float Tc = lm[wv][ec][4], Tci = 1.0f / Tc;
       const float isT = 1.0f / sqrtf(lm[wv][ec][4]);
 #pragma unroll
       for (int st = 0; st < 9; st++) {
@@ -184,11 +184,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             // raw_feature, read from the staged neighbourhood
             const int dx = nb / 9 - 1, dyz = nb - 9 * (nb / 9);  // (dy+1)*3 + (dz+1)
             const float* c = lm[wv][dyz * 66 + 16 * sub + col + 1 + dx];
-            const float raw = type == 0   ? c[0] * rci - 1.0f
+            // (relative differences as (v - v_c) / v_c: exactly zero where a neighbour equals
+            // the cell, so a uniform neighbourhood has all-zero features)
+            const float raw = type == 0   ? (c[0] - rc) * rci
                               : type == 1 ? (c[1] - uxc) * isT
                               : type == 2 ? (c[2] - uyc) * isT
                               : type == 3 ? (c[3] - uzc) * isT
-                                          : c[4] * Tci - 1.0f;
+                                          : (c[4] - Tc) * Tci;
             val = raw * w.in_inv[type < 5 ? type : 0];
           }
           v[e] = (short)f2bf(val);

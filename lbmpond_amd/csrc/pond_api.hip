@@ -548,8 +548,9 @@ pond_status pond_step(pond_ctx* ctx, int nsteps) {
     ctx->ev_pool.pop_back();
     const int in = ctx->cur, outb = 1 - ctx->cur;
     hipError_t he = hipSuccess;
-    if (ctx->split) {
-      he = launch_node_gauge(ctx->sp[in].m, ctx->sp[in].T, ctx->sq, ctx->g.n, ctx->g.plane, ctx->stream);
+    if (ctx->split && ctx->p.lattice == POND_D3Q27) {
+      he = launch_node_gauge(ctx->sp[in].f, ctx->sp[in].m, ctx->sp[in].T, ctx->sq, ctx->g.n, ctx->g.plane,
+                             ctx->stream);
       if (he != hipSuccess) return fail(ctx, POND_ERR_HIP, std::string("node gauge kernel: ") + hipGetErrorString(he));
     }
     const bool learned = ctx->split && ctx->p.use_mlp && ctx->mlp.trained;

@@ -45,8 +45,10 @@ struct SplitPtrs {
 
 using SplitPending = Pending;
 
-// node gauge records (ux, uy, uz, sqrt(|T|)*sign(T)), 16 bytes per cell, of a split state
-hipError_t launch_node_gauge(const float* m, const double* T, float* g4, int64_t n, int64_t plane, hipStream_t st);
+// node gauge records (ux, uy, uz, sqrt(|T|)*sign(T)), 16 bytes per cell, of a D3Q27 split state;
+// also sets its non-finite node data to the values that add only finite transfer terms
+hipError_t launch_node_gauge(float* f, const float* m, const double* T, float* g4, int64_t n, int64_t plane,
+                             hipStream_t st);
 hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const SplitPtrs& out, const float* g4,
                              int* niter, const float* guess, const SplitPending& pend, const SplitArgs& a,
                              hipStream_t st);

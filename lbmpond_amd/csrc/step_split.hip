@@ -197,10 +197,14 @@ __device__ __forceinline__ float weno5_combine(float c, const float ub[5], const
   const float b0 = (13.f / 12.f) * p0 * p0 + 0.25f * q0 * q0;
   const float b1 = (13.f / 12.f) * p1 * p1 + 0.25f * q1 * q1;
   const float b2 = (13.f / 12.f) * p2 * p2 + 0.25f * q2 * q2;
+  // Weights d_k / (e + b_k)^2, each multiplied by (e + b_min)^2: the same normalised weights,
+  // but every factor (e + b_min) / (e + b_k) lies in (0, 1], so large samples (b ~ 1e19 and up,
+  // where (e + b)^2 leaves FP32 range and all three weights would round to 0/0) keep finite
+  // weights with the largest one equal to d_k.
   const float e = 1e-6f;
-  const float w0 = d0 / ((e + b0) * (e + b0));
-  const float w1 = d1 / ((e + b1) * (e + b1));
-  const float w2 = d2 / ((e + b2) * (e + b2));
+  const float em = e + fminf(b0, fminf(b1, b2));
+  const float r0 = em / (e + b0), r1 = em / (e + b1), r2 = em / (e + b2);
+  const float w0 = d0 * (r0 * r0), w1 = d1 * (r1 * r1), w2 = d2 * (r2 * r2);
   return (w0 * S[0] + w1 * S[1] + w2 * S[2]) / (w0 + w1 + w2);
 }
 
@@ -429,8 +433,19 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
 
 // node gauge record of every cell for the node side of the transfer: (ux, uy, uz) and
 // sqrt(|T|)*sign(T) in FP32, 16 bytes per cell
-__global__ void k_node_gauge(const float* __restrict__ m, const double* __restrict__ T, float* __restrict__ g4,
-                             int64_t n, int64_t plane) {
+// Node gauge records of a D3Q27 split state, and its non-finite node data.  The reference
+// transfer drops every term w*g*f that is NaN or infinite (structs.cuh:542), so a node with
+// non-finite data adds only its finite terms to an interpolation instead of turning the whole
+// neighbourhood NaN.  With a finite gauge (finite basis values) a term is non-finite exactly
+// when its population is, and with a non-finite gauge every term of the node is.  So before the
+// step reads the state as nodes, a non-finite population is set to zero, and a node with a
+// non-finite gauge gets all-zero populations and a zero gauge record (zero gauge: finite basis
+// values, so the node adds nothing).  This rewrites the load state, which the step consumes: its
+// cells are updated from their own moments (m, T, unchanged here) and the rewritten populations
+// are read only as nodes.  FP32 ranges: a population or gauge beyond FP32 range is already
+// infinite in the split state.
+__global__ void k_node_gauge(float* __restrict__ f, const float* __restrict__ m, const double* __restrict__ T,
+                             float* __restrict__ g4, int64_t n, int64_t plane) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const double t = T[c];
@@ -439,6 +454,23 @@ __global__ void k_node_gauge(const float* __restrict__ m, const double* __restri
   v.y = m[2 * plane + c];
   v.z = m[3 * plane + c];
   v.w = (float)(sqrt(fabs(t)) * (double)((0.0 < t) - (t < 0.0)));
+  const bool gok = isfinite(v.x) && isfinite(v.y) && isfinite(v.z) && isfinite(v.w);
+  float4 q[7];
+  bool ok = gok;
+#pragma unroll
+  for (int j = 0; j < 7; j++) {
+    q[j] = reinterpret_cast<const float4*>(f)[j * plane + c];
+    ok = ok && isfinite(q[j].x) && isfinite(q[j].y) && isfinite(q[j].z) && (j == 6 || isfinite(q[j].w));
+  }
+  if (!ok) {
+    auto fix = [gok](float x) { return gok && isfinite(x) ? x : 0.0f; };
+#pragma unroll
+    for (int j = 0; j < 7; j++) {
+      float4 r = make_float4(fix(q[j].x), fix(q[j].y), fix(q[j].z), j == 6 ? q[j].w : fix(q[j].w));
+      reinterpret_cast<float4*>(f)[j * plane + c] = r;
+    }
+    if (!gok) v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
   reinterpret_cast<float4*>(g4)[c] = v;
 }
 
@@ -541,8 +573,11 @@ __device__ __forceinline__ bool split27_once(const NodeSrcF& src, float* __restr
     for (int k = 0; k < 4; k++) {
       const double scale = k < 3 ? sth : fabs(m[3]);
       const double d = fabs(m[k] - gc[k]);
-      if (!(d <= a.tol_rel * scale + a.tol_abs)) conv = false;
-      res = fmax(res, d / (scale + 1e-300));
+      // a NaN difference counts as converged, as in the reference's test (steps.cu:58-75,
+      // which fails only on difference >= tolerance), so a cell with non-finite data stops at once
+      if (d > a.tol_rel * scale + a.tol_abs) conv = false;
+      const double rk = d / (scale + 1e-300);
+      if (res == res && !(rk <= res)) res = rk;  // NaN propagates (fmax would drop it)
     }
     // FP32 populations leave a noise floor near the tolerance (interpolation rounding of a few
     // 1e-7 per population): a residual below 4 tol that has stopped falling has reached it.  A
@@ -553,11 +588,12 @@ __device__ __forceinline__ bool split27_once(const NodeSrcF& src, float* __restr
         (!st.learned || st.contracted))
       conv = true;
     if (st.niter - st.base > 1 && res < st.res_prev) st.contracted = true;
-    if (conv) {
+    // (a NaN iterate from a predicted start is not accepted: the cell falls back)
+    if (conv && !(st.learned && res != res)) {
       for (int k = 0; k < 4; k++) gc[k] = m[k];
       return true;
     }
-    if (st.learned && ((st.niter > 1 && !(res < st.res_prev)) || st.niter == kLearnedBudget)) {
+    if (st.learned && (res != res || (st.niter > 1 && !(res < st.res_prev)) || st.niter == kLearnedBudget)) {
       // the predicted start does not contract (or not within the budget): restart from the
       // previous gauge, so the cell ends where the plain method ends
       for (int k = 0; k < 4; k++) gc[k] = gp[k];
@@ -607,7 +643,7 @@ __device__ __forceinline__ void split27_finish(float* __restrict__ fout, float* 
 }
 
 // Starting state of a cell: its previous gauge, or the predicted one where a prediction is
-// given, finite, has T > 0 and lies within the trust radius.
+// given, finite, not all zero, has T > 0 and lies within the trust radius.
 __device__ __forceinline__ void split27_start(const float* __restrict__ min, const double* __restrict__ Tin,
                                               const float* __restrict__ guess, int64_t cell, double gp[4],
                                               SplitIter& st, const SplitArgs& a) {
@@ -627,7 +663,9 @@ __device__ __forceinline__ void split27_start(const float* __restrict__ min, con
     const float p0 = guess[cell], p1 = guess[P + cell], p2 = guess[2 * P + cell], p3 = guess[3 * P + cell];
     const double q[4] = {gp[0] + sq * p0, gp[1] + sq * p1, gp[2] + sq * p2, gp[3] * (1.0 + (double)p3)};
     const float pmax = fmaxf(fmaxf(fabsf(p0), fabsf(p1)), fmaxf(fabsf(p2), fabsf(p3)));
-    if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0 && pmax <= a.learned_trust) {
+    // (a prediction of no change at all is the previous-gauge start, with its stopping rules)
+    if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0 && pmax <= a.learned_trust &&
+        pmax > 0.0f) {
       for (int k = 0; k < 4; k++) st.gc[k] = q[k];
       st.learned = true;
     }
@@ -830,12 +868,19 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
   for (int n = 0; n < 27; n++) {
     const int px = wrapi(x - 1 + n / 9, g.nx), py = wrapi(y - 1 + (n / 3) % 3, g.ny), pz = wrapi(z - 1 + n % 3, g.nz);
     const int64_t c = cell_index(g, px, py, pz);
-    if (lane_ok) sf[n][k] = fin[(int64_t)k * P + c];
-    if (k < 3) su[n][k] = min[(int64_t)(k + 1) * P + c];
-    if (k == 3) {
-      const double t = Tin[c];
-      ss0[n] = (float)(sqrt(fabs(t)) * (double)((0.0 < t) - (t < 0.0)));
+    // non-finite data as in k_node_gauge (the reference drops non-finite transfer terms,
+    // structs.cuh:516): a non-finite population stages as zero, a node with a non-finite gauge
+    // as all zeros with a zero gauge
+    const float ux = min[P + c], uy = min[2 * P + c], uz = min[3 * P + c];
+    const double t = Tin[c];
+    const float s0 = (float)(sqrt(fabs(t)) * (double)((0.0 < t) - (t < 0.0)));
+    const bool gok = isfinite(ux) && isfinite(uy) && isfinite(uz) && isfinite(s0);
+    if (lane_ok) {
+      const float v = fin[(int64_t)k * P + c];
+      sf[n][k] = gok && isfinite(v) ? v : 0.0f;
     }
+    if (k < 3) su[n][k] = gok ? (k == 0 ? ux : k == 1 ? uy : uz) : 0.0f;
+    if (k == 3) ss0[n] = gok ? s0 : 0.0f;
   }
   double rho0 = min[cell];
   double gp[4] = {min[P + cell], min[2 * P + cell], min[3 * P + cell], Tin[cell]};
@@ -848,7 +893,8 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
                          gp[3] * (1.0 + (double)guess[3 * P + cell])};
     const float pmax = fmaxf(fmaxf(fabsf(guess[cell]), fabsf(guess[P + cell])),
                              fmaxf(fabsf(guess[2 * P + cell]), fabsf(guess[3 * P + cell])));
-    if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0 && pmax <= a.learned_trust) {
+    if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0 && pmax <= a.learned_trust &&
+        pmax > 0.0f) {
       for (int d = 0; d < 4; d++) gc[d] = q[d];
       learned = true;
     }
@@ -958,16 +1004,17 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
     for (int d = 0; d < 4; d++) {
       const double scale = d < 3 ? st : fabs(m[3]);
       const double dd = fabs(m[d] - gc[d]);
-      if (!(dd <= a.tol_rel * scale + a.tol_abs)) conv = false;
-      res = fmax(res, dd / (scale + 1e-300));
+      if (dd > a.tol_rel * scale + a.tol_abs) conv = false;  // NaN: converged, as in the D3Q27 step
+      const double rk = dd / (scale + 1e-300);
+      if (res == res && !(rk <= res)) res = rk;  // NaN propagates (fmax would drop it)
     }
     if (!conv && niter - base > 1 && res <= 4.0 * a.tol_rel && res >= res_prev && (!learned || contracted)) conv = true;
     if (niter - base > 1 && res < res_prev) contracted = true;
-    if (conv) {
+    if (conv && !(learned && res != res)) {
       for (int d = 0; d < 4; d++) gc[d] = m[d];
       break;
     }
-    if (learned && ((niter > 1 && !(res < res_prev)) || niter == kLearnedBudget)) {
+    if (learned && (res != res || (niter > 1 && !(res < res_prev)) || niter == kLearnedBudget)) {
       for (int d = 0; d < 4; d++) gc[d] = gp[d];
       learned = false;
       contracted = false;
@@ -990,8 +1037,9 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
   }
 }
 
-hipError_t launch_node_gauge(const float* m, const double* T, float* g4, int64_t n, int64_t plane, hipStream_t st) {
-  hipLaunchKernelGGL(k_node_gauge, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, m, T, g4, n, plane);
+hipError_t launch_node_gauge(float* f, const float* m, const double* T, float* g4, int64_t n, int64_t plane,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(k_node_gauge, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, f, m, T, g4, n, plane);
   return hipGetLastError();
 }
 

@@ -199,3 +199,39 @@ def test_split_absurd_gauges_terminate(gpu):
     print("  split vs fp64 away from the absurd cells:", {k: f"{v:.2e}" for k, v in errs.items()})
     for k in STATE_ERRORS:
         assert errs[k] <= 1e-5, (k, errs[k])
+
+
+@pytest.mark.parametrize("lattice,interp,shape", [("D3Q27", "weno5c", (32, 12, 10)), ("D3Q125", "lagr4", (12, 10, 8))])
+def test_split_nonfinite_nodes_match_fp64(gpu, lattice, interp, shape):
+    """Non-finite data in a few cells (NaN or infinite temperature, density or velocity, and a
+    velocity beyond FP32 range).  The reference transfer drops non-finite terms
+    (structs.cuh:542), which confines such a cell to itself: its neighbours interpolate it as a
+    node that adds only finite terms.  The split step must confine it the same way: outside the
+    influence of iteration-capped cells the non-finite cells coincide with the FP64 step's, and
+    the finite ones match it to 1e-5."""
+    nx, ny, nz = shape
+    rho, ux, uy, uz, T = smooth_random_state(shape, seed=75, T0=0.02)
+    def cell(x, y, z):
+        return x + nx * (y + ny * z)
+    T[cell(1, 2, 3)] = np.nan
+    ux[cell(2, 5, 5)] = 1e300
+    T[cell(3, 9 % ny, 1)] = np.inf
+    uz[cell(nx // 2, 4, 4)] = -np.inf
+    rho[cell(nx - 5, 7, nz - 2)] = np.nan
+    fields = (rho, ux, uy, uz, T)
+    steps = 2
+    ref = run(lattice, shape, interp, "fp64", fields, steps)
+    got = run(lattice, shape, interp, "split", fields, steps)
+    dep = ref["cap_dependence"].mask | got["cap_dependence"].mask
+    bad_ref, bad_got = ~np.isfinite(ref["rho"]), ~np.isfinite(got["rho"])
+    print(f"  non-finite cells fp64 {np.flatnonzero(bad_ref)} split {np.flatnonzero(bad_got)};"
+          f" {dep.mean():.3f} of the cells cap-dependent", flush=True)
+    assert bad_ref.sum() <= 5, "the reference confines each non-finite cell"
+    assert np.array_equal(bad_ref & ~dep, bad_got & ~dep)
+    good = ~dep & ~bad_ref
+    assert good.mean() > 0.5
+    keys = ("rho", "ux", "uy", "uz", "T")
+    errs = state_errors({k: got[k][good] for k in keys}, {k: ref[k][good] for k in keys})
+    print("  split vs fp64 on the finite cells:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k in STATE_ERRORS:
+        assert errs[k] <= 1e-5, (k, errs[k])
