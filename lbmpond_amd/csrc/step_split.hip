@@ -329,13 +329,13 @@ __device__ void interp_group_weno5c(const NodeSrcF& src, int x, int y, int z, do
     }
     return;
   }
-  float S[3][3], ub[3][5];
+  // per target and window position j: the middle node's value (smoothness samples) and the
+  // slice's y/z sum
+  float ub[3][5], yv[3][5];
 #pragma unroll
   for (int t = 0; t < 3; t++) {
 #pragma unroll
-    for (int k = 0; k < 3; k++) S[t][k] = 0.f;
-#pragma unroll
-    for (int j = 0; j < 5; j++) ub[t][j] = 0.f;
+    for (int j = 0; j < 5; j++) ub[t][j] = yv[t][j] = 0.f;
   }
   // contribution of one stencil node (its moments and 27 populations) to the three targets'
   // y/z sums yz (weight wab) and, for the middle node, to their smoothness samples mid
@@ -368,24 +368,17 @@ __device__ void interp_group_weno5c(const NodeSrcF& src, int x, int y, int z, do
     }
   };
   // slice xs (its y/z sums yz and middle values mid) enters target t's window at position
-  // j = xs - (r_t - 2)
+  // j = xs - (r_t - 2); the sub-stencil sums are formed once the window is complete
   auto slice = [&](int xs, const float yz[3], const float mid[3]) {
 #pragma unroll
     for (int t = 0; t < 3; t++) {
       const int j = xs - (r[t] - 2);
+      // (selects with constant indices: the arrays stay in registers)
 #pragma unroll
       for (int jj = 0; jj < 5; jj++) {
-        if (j != jj) continue;
-        ub[t][jj] = mid[t];
-        // node jj - k of sub-stencil k (left node r-2+k), quadratic Lagrange weight at c + 2 - k
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          const int m = jj - k;
-          if (m < 0 || m > 2) continue;
-          const float sk = c[t] + (float)(2 - k);
-          const float wk = m == 0 ? 0.5f * (sk - 1.0f) * (sk - 2.0f) : m == 1 ? -sk * (sk - 2.0f) : 0.5f * sk * (sk - 1.0f);
-          S[t][k] = __builtin_fmaf(wk, yz[t], S[t][k]);
-        }
+        const bool at = j == jj;
+        ub[t][jj] = at ? mid[t] : ub[t][jj];
+        yv[t][jj] = at ? yz[t] : yv[t][jj];
       }
     }
   };
@@ -420,8 +413,19 @@ __device__ void interp_group_weno5c(const NodeSrcF& src, int x, int y, int z, do
     slice(xs, yz, mid);
     if (two) slice(xs + 1, yz2, mid2);
   }
+  // sub-stencil k (left node r-2+k): quadratic Lagrange weights at c + 2 - k on its nodes
+  // m = 0, 1, 2, summed in node order
 #pragma unroll
-  for (int t = 0; t < 3; t++) out[t] = weno5_combine(c[t], ub[t], S[t]);
+  for (int t = 0; t < 3; t++) {
+    float S[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const float sk = c[t] + (float)(2 - k);
+      const float w0 = 0.5f * (sk - 1.0f) * (sk - 2.0f), w1 = -sk * (sk - 2.0f), w2 = 0.5f * sk * (sk - 1.0f);
+      S[k] = __builtin_fmaf(w2, yv[t][k + 2], __builtin_fmaf(w1, yv[t][k + 1], __builtin_fmaf(w0, yv[t][k], 0.f)));
+    }
+    out[t] = weno5_combine(c[t], ub[t], S);
+  }
 }
 
 __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
