@@ -64,6 +64,15 @@ __device__ __forceinline__ void basis3(float D, float sq0, bool kn, float c, flo
   }
 }
 
+// basis3 without the per-axis factor w (basis3 = w * basis3g, rounded once more)
+__device__ __forceinline__ void basis3g(float D, float sq0, bool kn, float c, float T1, float G[3]) {
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    const float A = D - (float)(j - 1) * sq0;
+    G[j] = kn ? A * (A - c) : __builtin_fmaf(A, A, -T1);
+  }
+}
+
 // Buffer resources: a load names its plane (or quad plane) through the scalar offset and the
 // per-lane address is a single 32-bit offset, where a global load needs a 64-bit per-lane
 // address (two VGPRs and a 64-bit add per load).  The resource size is exact, so the hardware
@@ -341,9 +350,11 @@ __device__ void interp_group_weno5c(const NodeSrcF& src, int x, int y, int z, do
   // y/z sums yz (weight wab) and, for the middle node, to their smoothness samples mid
   auto node = [&](float u0x, float u0y, float u0z, float sq0, const float f[27], float wab, bool is_mid, float yz[3],
                   float mid[3]) {
+    // basis values without their per-axis factors w: those are constants of the group (y, z)
+    // and of the target (x), applied once to the finished window values
     float Ly[3], Lz[3];
-    basis3(((u1y - u0y) + u1yl) * sqTb, sq0, ky, cyb, T1, wy, Ly);
-    basis3(((u1z - u0z) + u1zl) * sqTb, sq0, kz, czb, T1, wz, Lz);
+    basis3g(((u1y - u0y) + u1yl) * sqTb, sq0, ky, cyb, T1, Ly);
+    basis3g(((u1z - u0z) + u1zl) * sqTb, sq0, kz, czb, T1, Lz);
     float sy[3];
 #pragma unroll
     for (int ja = 0; ja < 3; ja++) {
@@ -361,7 +372,7 @@ __device__ void interp_group_weno5c(const NodeSrcF& src, int x, int y, int z, do
 #pragma unroll
     for (int t = 0; t < 3; t++) {
       float Lx[3];
-      basis3(Dx, sq0, t != 1, (float)(t - 1) * sq1, T1, t != 1 ? h : mm, Lx);
+      basis3g(Dx, sq0, t != 1, (float)(t - 1) * sq1, T1, Lx);
       const float T = __builtin_fmaf(Lx[2], sy[2], __builtin_fmaf(Lx[1], sy[1], Lx[0] * sy[0]));
       yz[t] = __builtin_fmaf(wab, T, yz[t]);
       if (is_mid) mid[t] = T;
@@ -414,9 +425,18 @@ __device__ void interp_group_weno5c(const NodeSrcF& src, int x, int y, int z, do
     if (two) slice(xs + 1, yz2, mid2);
   }
   // sub-stencil k (left node r-2+k): quadratic Lagrange weights at c + 2 - k on its nodes
-  // m = 0, 1, 2, summed in node order
+  // m = 0, 1, 2, summed in node order; the window values first take the basis factors
+  // w_x(t) w_y w_z left out of the node contributions (the interpolation is linear in them, the
+  // WENO weights are not, so they enter before the smoothness indicators)
+  const float wyz = wy * wz;
 #pragma unroll
   for (int t = 0; t < 3; t++) {
+    const float wt = (t != 1 ? h : mm) * wyz;
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+      ub[t][j] *= wt;
+      yv[t][j] *= wt;
+    }
     float S[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
