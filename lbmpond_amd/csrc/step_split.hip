@@ -64,13 +64,15 @@ __device__ __forceinline__ void basis3(float D, float sq0, bool kn, float c, flo
   }
 }
 
-// basis3 without the per-axis factor w (basis3 = w * basis3g, rounded once more)
-__device__ __forceinline__ void basis3g(float D, float sq0, bool kn, float c, float T1, float G[3]) {
-#pragma unroll
-  for (int j = 0; j < 3; j++) {
-    const float A = D - (float)(j - 1) * sq0;
-    G[j] = kn ? A * (A - c) : __builtin_fmaf(A, A, -T1);
-  }
+// basis3 without the per-axis factor w, the axis kind folded into (c, T): a moving axis
+// (kn) passes c = e*sqrt(T1), T = 0 and gets A (A - c); a resting one passes c = 0, T = T1 and
+// gets A^2 - T1.  fma(A, A - c, -T) is either, rounded as in basis3, without a select between
+// the two forms where the kind is only known at run time.  A_j = D - (j - 1) sq0.
+__device__ __forceinline__ void basis3u(float D, float sq0, float c, float T, float G[3]) {
+  const float A0 = D + sq0, A1 = D, A2 = D - sq0;
+  G[0] = __builtin_fmaf(A0, A0 - c, -T);
+  G[1] = __builtin_fmaf(A1, A1 - c, -T);
+  G[2] = __builtin_fmaf(A2, A2 - c, -T);
 }
 
 // Buffer resources: a load names its plane (or quad plane) through the scalar offset and the
@@ -310,11 +312,18 @@ __device__ void interp_group_weno5c(const NodeSrcF& src, int x, int y, int z, do
   float Fy[3], Fz[3];
   lag3f((float)(cy - ry), Fy);
   lag3f((float)(cz - rz), Fz);
-  int yy[3], zz[3];
+  // row offsets of the stencil rows: node (x, a, b) is cell x + yo[a] + zo[b] (the split
+  // path's grids have 32-bit cell indices)
+  uint32_t yo[3], zo[3];
 #pragma unroll
-  for (int a = 0; a < 3; a++) { yy[a] = wrapi(ry + a, g.ny); zz[a] = wrapi(rz + a, g.nz); }
-  const bool ky = ey != 0, kz = ez != 0;
-  const float cyb = (float)ey * sq1, czb = (float)ez * sq1, wy = ey ? h : mm, wz = ez ? h : mm;
+  for (int a = 0; a < 3; a++) {
+    yo[a] = (uint32_t)wrapi(ry + a, g.ny) * (uint32_t)g.nx;
+    zo[a] = (uint32_t)wrapi(rz + a, g.nz) * ((uint32_t)g.nx * (uint32_t)g.ny);
+  }
+  // y and z basis kinds (basis3u) and factors
+  const float cyb = ey ? (float)ey * sq1 : 0.0f, Tyb = ey ? 0.0f : T1;
+  const float czb = ez ? (float)ez * sq1 : 0.0f, Tzb = ez ? 0.0f : T1;
+  const float wy = ey ? h : mm, wz = ez ? h : mm;
   // x: per target t (ex = t - 1) its window r_t-2 .. r_t+2 and sub-stencil weights
   int r[3];
   float c[3];  // cx - r: sub-stencil k evaluates its quadratic at c + 2 - k
@@ -353,8 +362,8 @@ __device__ void interp_group_weno5c(const NodeSrcF& src, int x, int y, int z, do
     // basis values without their per-axis factors w: those are constants of the group (y, z)
     // and of the target (x), applied once to the finished window values
     float Ly[3], Lz[3];
-    basis3g(((u1y - u0y) + u1yl) * sqTb, sq0, ky, cyb, T1, Ly);
-    basis3g(((u1z - u0z) + u1zl) * sqTb, sq0, kz, czb, T1, Lz);
+    basis3u(((u1y - u0y) + u1yl) * sqTb, sq0, cyb, Tyb, Ly);
+    basis3u(((u1z - u0z) + u1zl) * sqTb, sq0, czb, Tzb, Lz);
     float sy[3];
 #pragma unroll
     for (int ja = 0; ja < 3; ja++) {
@@ -368,14 +377,20 @@ __device__ void interp_group_weno5c(const NodeSrcF& src, int x, int y, int z, do
       }
       sy[ja] = acc;
     }
+    // x basis of the three targets from shared sums: with A_j = Dx - (j - 1) sq0 the moving
+    // targets (ex = -1, +1) have g_j = A_j (A_j + sq1), A_j (A_j - sq1) and the resting one
+    // A_j^2 - T1, so with P = sum A_j^2 sy_j, Q = sum A_j sy_j, R = sum sy_j the three values
+    // are P + sq1 Q, P - T1 R and P - sq1 Q
     const float Dx = ((u1x - u0x) + u1xl) * sqTb;
+    const float A0 = Dx + sq0, A1 = Dx, A2 = Dx - sq0;
+    const float Q = __builtin_fmaf(A2, sy[2], __builtin_fmaf(A1, sy[1], A0 * sy[0]));
+    const float P = __builtin_fmaf(A2, A2 * sy[2], __builtin_fmaf(A1, A1 * sy[1], A0 * (A0 * sy[0])));
+    const float R = (sy[0] + sy[1]) + sy[2];
+    const float Tt[3] = {__builtin_fmaf(sq1, Q, P), __builtin_fmaf(-T1, R, P), __builtin_fmaf(-sq1, Q, P)};
 #pragma unroll
     for (int t = 0; t < 3; t++) {
-      float Lx[3];
-      basis3g(Dx, sq0, t != 1, (float)(t - 1) * sq1, T1, Lx);
-      const float T = __builtin_fmaf(Lx[2], sy[2], __builtin_fmaf(Lx[1], sy[1], Lx[0] * sy[0]));
-      yz[t] = __builtin_fmaf(wab, T, yz[t]);
-      if (is_mid) mid[t] = T;
+      yz[t] = __builtin_fmaf(wab, Tt[t], yz[t]);
+      if (is_mid) mid[t] = Tt[t];
     }
   };
   // slice xs (its y/z sums yz and middle values mid) enters target t's window at position
@@ -404,18 +419,18 @@ __device__ void interp_group_weno5c(const NodeSrcF& src, int x, int y, int z, do
 #pragma unroll 1
     for (int q = 0; q < 9; q++) {
       const int a = q / 3, b = q - 3 * (q / 3);
-      const int ya = a == 0 ? yy[0] : a == 1 ? yy[1] : yy[2];
-      const int zb = b == 0 ? zz[0] : b == 1 ? zz[1] : zz[2];
+      const uint32_t ya = a == 0 ? yo[0] : a == 1 ? yo[1] : yo[2];
+      const uint32_t zb = b == 0 ? zo[0] : b == 1 ? zo[1] : zo[2];
       const float fya = a == 0 ? Fy[0] : a == 1 ? Fy[1] : Fy[2];
       const float fzb = b == 0 ? Fz[0] : b == 1 ? Fz[1] : Fz[2];
       const float wab = fya * fzb;
-      const uint32_t boff = (uint32_t)cell_index(g, xw, ya, zb) * 4u;
+      const uint32_t boff = ((uint32_t)xw + ya + zb) * 4u;
       float f[27];
       load27(src.f, boff, f);
       const Gauge4 n = ldg4(src, boff);
       node(n.ux, n.uy, n.uz, n.sq, f, wab, q == 4, yz, mid);
       if (two) {
-        const uint32_t boff2 = (uint32_t)cell_index(g, xw2, ya, zb) * 4u;
+        const uint32_t boff2 = ((uint32_t)xw2 + ya + zb) * 4u;
         load27(src.f, boff2, f);
         const Gauge4 n2 = ldg4(src, boff2);
         node(n2.ux, n2.uy, n2.uz, n2.sq, f, wab, q == 4, yz2, mid2);
