@@ -148,14 +148,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       y = (int)(row % g.ny);
       z = (int)(row / g.ny);
       wave_sync();  // the previous segment's reads of this wave's LDS are done
-      for (int e = l; e < kStage; e += 64) {
-        const int xx = e % 66, r = e / 66;
-        const int64_t c =
-            cell_index(g, wrapi(xs0 - 1 + xx, g.nx), wrapi(y + r / 3 - 1, g.ny), wrapi(z + r % 3 - 1, g.nz));
+      // Entries e = l + 64 i.  The loop is unrolled and its indices wrap without branches
+      // (wrap2: one period either way, which the y and z rows always are and the x positions
+      // are on rows of 65 cells or more), so a lane's loads of all its entries can be in
+      // flight together; shorter rows take wrapi, whose slow path branches.
+      auto wrap2 = [](int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); };
+      auto stage = [&](auto wrapx) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) lm[wv][e][k] = s.m[(int64_t)k * g.plane + c];
-        lm[wv][e][4] = (float)s.T[c];
-      }
+        for (int i = 0; i < (kStage + 63) / 64; i++) {
+          const int e = l + 64 * i;
+          if (e < kStage) {
+            const int xx = e % 66, r = e / 66;
+            const int64_t c = cell_index(g, wrapx(xs0 - 1 + xx), wrap2(y + r / 3 - 1, g.ny), wrap2(z + r % 3 - 1, g.nz));
+#pragma unroll
+            for (int k = 0; k < 4; k++) lm[wv][e][k] = s.m[(int64_t)k * g.plane + c];
+            lm[wv][e][4] = (float)s.T[c];
+          }
+        }
+      };
+      if (g.nx >= 65) stage([&](int v) { return wrap2(v, g.nx); });
+      else stage([&](int v) { return wrapi(v, g.nx); });
       wave_sync();
     }
 #pragma unroll 1
@@ -179,20 +191,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         for (int e = 0; e < 4; e++) {
           const int j = st * 4 + e;
           const int type = j / 7, nb = (j % 7) * 4 + q;  // slot_feature: type fixed per (st, e)
-          float val = 0.0f;
-          if (j < 35 && live && nb < 27) {
-            // raw_feature, read from the staged neighbourhood
-            const int dx = nb / 9 - 1, dyz = nb - 9 * (nb / 9);  // (dy+1)*3 + (dz+1)
-            const float* c = lm[wv][dyz * 66 + 16 * sub + col + 1 + dx];
-            // (relative differences as (v - v_c) / v_c: exactly zero where a neighbour equals
-            // the cell, so a uniform neighbourhood has all-zero features)
-            const float raw = type == 0   ? (c[0] - rc) * rci
-                              : type == 1 ? (c[1] - uxc) * isT
-                              : type == 2 ? (c[2] - uyc) * isT
-                              : type == 3 ? (c[3] - uzc) * isT
-                                          : (c[4] - Tc) * Tci;
-            val = raw * w.in_inv[type < 5 ? type : 0];
-          }
+          // raw_feature, read from the staged neighbourhood.  Every lane reads (slot 26 in
+          // place of the unused slots 27, 28) and the unused values are zeroed afterwards, so
+          // the reads carry no branch and a lane's 36 LDS reads can be in flight together.
+          const int nbc = nb < 27 ? nb : 26;
+          const int dx = nbc / 9 - 1, dyz = nbc - 9 * (nbc / 9);  // (dy+1)*3 + (dz+1)
+          const float cv = lm[wv][dyz * 66 + 16 * sub + col + 1 + dx][type < 5 ? type : 0];
+          // (relative differences as (v - v_c) / v_c: exactly zero where a neighbour equals
+          // the cell, so a uniform neighbourhood has all-zero features)
+          const float raw = type == 0   ? (cv - rc) * rci
+                            : type == 1 ? (cv - uxc) * isT
+                            : type == 2 ? (cv - uyc) * isT
+                            : type == 3 ? (cv - uzc) * isT
+                                        : (cv - Tc) * Tci;
+          const float val = (j < 35 && live && nb < 27) ? raw * w.in_inv[type < 5 ? type : 0] : 0.0f;
           v[e] = (short)f2bf(val);
         }
         xb[st] = v;
