@@ -31,6 +31,10 @@ __device__ __forceinline__ unsigned short f2bf(float f) {
 
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
 
+// round to nearest even in one instruction (v_cvt_pk_bf16_f32); the same as f2bf for every
+// finite value
+__device__ __forceinline__ short f2bf_hw(float f) { return (short)__builtin_bit_cast(unsigned short, (__bf16)f); }
+
 __device__ __forceinline__ float4_t mfma16(short4_t a, short4_t b, float4_t c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
@@ -184,19 +188,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       const float rc = lm[wv][ec][0], rci = 1.0f / rc, uxc = lm[wv][ec][1], uyc = lm[wv][ec][2], uzc = lm[wv][ec][3];
       const float Tc = lm[wv][ec][4], Tci = 1.0f / Tc;
       const float isT = 1.0f / sqrtf(lm[wv][ec][4]);
+      const int lane_base = 66 * q + 16 * sub + col;
 #pragma unroll
       for (int st = 0; st < 9; st++) {
         short4_t v;
 #pragma unroll
         for (int e = 0; e < 4; e++) {
           const int j = st * 4 + e;
-          const int type = j / 7, nb = (j % 7) * 4 + q;  // slot_feature: type fixed per (st, e)
-          // raw_feature, read from the staged neighbourhood.  Every lane reads (slot 26 in
-          // place of the unused slots 27, 28) and the unused values are zeroed afterwards, so
-          // the reads carry no branch and a lane's 36 LDS reads can be in flight together.
-          const int nbc = nb < 27 ? nb : 26;
-          const int dx = nbc / 9 - 1, dyz = nbc - 9 * (nbc / 9);  // (dy+1)*3 + (dz+1)
-          const float cv = lm[wv][dyz * 66 + 16 * sub + col + 1 + dx][type < 5 ? type : 0];
+          const int type = j / 7, m = j % 7;  // slot_feature: type fixed per (st, e)
+          // raw_feature of neighbour nb = 4m + q, read from the staged neighbourhood at entry
+          // (nb mod 9) * 66 + 16 sub + col + nb / 9 = 66 nb - 593 (nb / 9) + 16 sub + col.
+          // With q in 0..3, nb / 9 is k0 = 4m / 9 or k0 + 1 (from q >= t), so the entry is a
+          // per-lane base plus one of two constants.  Every lane reads (slots 27, 28 land on
+          // staged entries too) and the unused values are zeroed afterwards, so the reads
+          // carry no branch and a lane's LDS reads can be in flight together.
+          const int k0 = (4 * m) / 9, t = 9 * (k0 + 1) - 4 * m;  // constants once unrolled
+          const int c0 = 264 * m - 593 * k0;
+          const int nb = 4 * m + q;
+          const float cv = lm[wv][lane_base + (q >= t ? c0 - 593 : c0)][type < 5 ? type : 0];
           // (relative differences as (v - v_c) / v_c: exactly zero where a neighbour equals
           // the cell, so a uniform neighbourhood has all-zero features)
           const float raw = type == 0   ? (cv - rc) * rci
@@ -205,7 +214,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                             : type == 3 ? (cv - uzc) * isT
                                         : (cv - Tc) * Tci;
           const float val = (j < 35 && live && nb < 27) ? raw * w.in_inv[type < 5 ? type : 0] : 0.0f;
-          v[e] = (short)f2bf(val);
+          v[e] = f2bf_hw(val);
         }
         xb[st] = v;
       }
@@ -232,7 +241,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       for (int st = 0; st < 9; st++) acc = mfma16(a1[t][st], xb[st], acc);
       short4_t hv;
 #pragma unroll
-      for (int e = 0; e < 4; e++) hv[e] = (short)f2bf(fmaxf(acc[e] + b1[t][e], 0.0f));
+      for (int e = 0; e < 4; e++) hv[e] = f2bf_hw(fmaxf(acc[e] + b1[t][e], 0.0f));
       h1[t] = hv;
     }
     // layer 2
@@ -244,7 +253,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       for (int st = 0; st < 4; st++) acc = mfma16(a2[t][st], h1[st], acc);
       short4_t hv;
 #pragma unroll
-      for (int e = 0; e < 4; e++) hv[e] = (short)f2bf(fmaxf(acc[e] + b2[t][e], 0.0f));
+      for (int e = 0; e < 4; e++) hv[e] = f2bf_hw(fmaxf(acc[e] + b2[t][e], 0.0f));
       h2[t] = hv;
     }
     // layer 3: rows 0..3 of the output tile live in lanes 0..15 (q == 0)
