@@ -297,7 +297,8 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
   if (const char* v = std::getenv("POND_NO_BUF")) a.no_buf = std::atoi(v) != 0;
   // lane refill in the first pass of the FP64 D3Q27 step (POND_REFILL_CHUNK_F64 overrides, 0 =
   // one tile per wave).  384^3 shock-bubble FP64 step: one tile per wave 8.07 s, chunks of 2
-  // tiles 6.38 s, 4 tiles 7.05 s (4 tiles in a single pass 6.90 s).
+  // tiles 6.38 s, 4 tiles 7.05 s (4 tiles in a single pass 6.90 s); with 4x4 row patches
+  // 2 tiles 2.94 s, 3 3.06 s, 6 3.09 s.
   a.refill_chunk = 2;
   if (const char* v = std::getenv("POND_REFILL_CHUNK_F64")) a.refill_chunk = std::atoi(v) > 0 ? std::atoi(v) : 0;
   // Iterations of the first pass of the two-pass step; cells still unconverged are finished by
@@ -351,8 +352,17 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
     // wave).  Measured on the 384^3 shock-bubble, split step ms, with paired x-slice loads: one
     // tile 1065, chunks of 2 tiles 805, 3 976, 4 775-778, 6 763, 8 840, 12 910 (one slice per
     // load and one tile per wave: 1025).  Larger chunks keep more lanes busy but spread the
-    // active cells of a wave over more rows, so its loads touch more cache lines.
+    // active cells of a wave over more rows, so its loads touch more cache lines.  With the
+    // current node loop (gather-bound): 3 319, 4 293, 5 305, 6 279, 7 321, 8 317, 9 328, 12 321.
+    // Six tiles are one whole x row at 384^3: with row patches the tiles are 64-cell row
+    // segments, nx/64 per row, so chunks of whole rows let the waves resident on an XCD cover
+    // a compact patch of rows (traffic from beyond L2 per launch: 1.23 TB with chunks of 4
+    // tiles, 0.50 TB with 6).  Default: whole rows, at least 4 tiles.
     sa.refill_chunk = 4;
+    if (g.rp > 0) {
+      const int wpr = g.nx / 64;
+      sa.refill_chunk = wpr * ((4 + wpr - 1) / wpr);
+    }
     if (const char* v = std::getenv("POND_REFILL_CHUNK")) sa.refill_chunk = std::atoi(v) > 0 ? std::atoi(v) : 0;
     // largest predicted change taken as a start (POND_LEARNED_TRUST overrides).  384^3
     // shock-bubble, learned path: radius 0.1 1.290 mean iterations, 307 ms step kernel; 0.3 and
