@@ -35,6 +35,17 @@ __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float
 // finite value
 __device__ __forceinline__ short f2bf_hw(float f) { return (short)__builtin_bit_cast(unsigned short, (__bf16)f); }
 
+// four values as one bf16 MFMA operand, two conversions per instruction (v_cvt_pk_bf16_f32)
+typedef float f2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf2_t __attribute__((ext_vector_type(2)));
+struct U32x2 {
+  unsigned int lo, hi;
+};
+__device__ __forceinline__ short4_t pack_bf16x4(float a, float b, float c, float d) {
+  const bf2_t lo = __builtin_convertvector((f2_t){a, b}, bf2_t), hi = __builtin_convertvector((f2_t){c, d}, bf2_t);
+  return __builtin_bit_cast(short4_t, (U32x2{__builtin_bit_cast(unsigned int, lo), __builtin_bit_cast(unsigned int, hi)}));
+}
+
 __device__ __forceinline__ float4_t mfma16(short4_t a, short4_t b, float4_t c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
@@ -189,9 +200,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       const float Tc = lm[wv][ec][4], Tci = 1.0f / Tc;
       const float isT = 1.0f / sqrtf(lm[wv][ec][4]);
       const int lane_base = 66 * q + 16 * sub + col;
+      // per feature type: reference value and factor (the cell's scale times the fitted one)
+      const float ref[5] = {rc, uxc, uyc, uzc, Tc};
+      const float mul[5] = {rci * w.in_inv[0], isT * w.in_inv[1], isT * w.in_inv[2], isT * w.in_inv[3],
+                            Tci * w.in_inv[4]};
 #pragma unroll
       for (int st = 0; st < 9; st++) {
-        short4_t v;
+        float v[4];
 #pragma unroll
         for (int e = 0; e < 4; e++) {
           const int j = st * 4 + e;
@@ -208,15 +223,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
           const float cv = lm[wv][lane_base + (q >= t ? c0 - 593 : c0)][type < 5 ? type : 0];
           // (relative differences as (v - v_c) / v_c: exactly zero where a neighbour equals
           // the cell, so a uniform neighbourhood has all-zero features)
-          const float raw = type == 0   ? (cv - rc) * rci
-                            : type == 1 ? (cv - uxc) * isT
-                            : type == 2 ? (cv - uyc) * isT
-                            : type == 3 ? (cv - uzc) * isT
-                                        : (cv - Tc) * Tci;
-          const float val = (j < 35 && live && nb < 27) ? raw * w.in_inv[type < 5 ? type : 0] : 0.0f;
-          v[e] = f2bf_hw(val);
+          const int ty = type < 5 ? type : 0;
+          const float raw = (cv - ref[ty]) * mul[ty];
+          v[e] = (j < 35 && live && nb < 27) ? raw : 0.0f;
         }
-        xb[st] = v;
+        xb[st] = pack_bf16x4(v[0], v[1], v[2], v[3]);
       }
     } else {
       cell = unit * 16 + col;
@@ -239,10 +250,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       float4_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int st = 0; st < 9; st++) acc = mfma16(a1[t][st], xb[st], acc);
-      short4_t hv;
-#pragma unroll
-      for (int e = 0; e < 4; e++) hv[e] = f2bf_hw(fmaxf(acc[e] + b1[t][e], 0.0f));
-      h1[t] = hv;
+      h1[t] = pack_bf16x4(fmaxf(acc[0] + b1[t][0], 0.0f), fmaxf(acc[1] + b1[t][1], 0.0f),
+                          fmaxf(acc[2] + b1[t][2], 0.0f), fmaxf(acc[3] + b1[t][3], 0.0f));
     }
     // layer 2
     short4_t h2[4];
@@ -251,10 +260,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       float4_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int st = 0; st < 4; st++) acc = mfma16(a2[t][st], h1[st], acc);
-      short4_t hv;
-#pragma unroll
-      for (int e = 0; e < 4; e++) hv[e] = f2bf_hw(fmaxf(acc[e] + b2[t][e], 0.0f));
-      h2[t] = hv;
+      h2[t] = pack_bf16x4(fmaxf(acc[0] + b2[t][0], 0.0f), fmaxf(acc[1] + b2[t][1], 0.0f),
+                          fmaxf(acc[2] + b2[t][2], 0.0f), fmaxf(acc[3] + b2[t][3], 0.0f));
     }
     // layer 3: rows 0..3 of the output tile live in lanes 0..15 (q == 0)
     float4_t acc = {0.f, 0.f, 0.f, 0.f};
