@@ -1049,10 +1049,9 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
     }
     if (!conv && niter - base > 1 && res <= 4.0 * a.tol_rel && res >= res_prev && (!learned || contracted)) conv = true;
     if (niter - base > 1 && res < res_prev) contracted = true;
-    // (unlike the D3Q27 step, a NaN residual from a predicted start is not a reason to fall
-    // back here: on the D3Q125 turbulence such transient residuals are common and the cells
-    // still converge from the predicted start; falling back on them doubled the mean
-    // iterations at 256^3, 2.35 -> 4.58)
+    // (unlike the D3Q27 step, no fallback from a predicted start on a NaN residual: with that
+    // rule the 256^3 D3Q125 turbulence took 4.58 mean iterations instead of 2.35, although a
+    // probe counted no NaN residual there; the cause is not found)
     if (conv) {
       for (int d = 0; d < 4; d++) gc[d] = m[d];
       break;
