@@ -74,6 +74,28 @@ typedef struct {
   int use_mlp;      /* 1: learned starting gauge (POND_SPLIT only) */
   double split_tol_rel; /* POND_SPLIT: relative tolerance on the FP64 gauge residual */
   double split_tol_abs; /* POND_SPLIT: absolute tolerance on the FP64 gauge residual */
+  /* Work distribution of the D3Q27 step kernels.  None of these changes a result (each is
+     tested bit for bit against the defaults); pond_params_default sets the measured defaults.
+     tile_patch    cells are taken in 64-cell x-row segments whose (y, z) rows are visited in
+                   tile_patch x tile_patch patches, so the waves resident on one XCD share
+                   their stencil halo in its L2; used where nx % 64 == 0 and tile_patch divides
+                   ny and nz.  0: plain row order.  Default 4.
+     refill_chunk  64-cell tiles owned by one wave, whose lanes take the next unstarted cell
+                   of the chunk when theirs converges.  0: one tile per wave.  -1 (default):
+                   FP64 2 tiles; split the whole x row with at least 4 tiles.
+     pass1_iters   iterations of the first pass of the two-pass step; cells not converged by
+                   then are finished by a second kernel over a compacted list.  -1 (default):
+                   FP64 8, split one pass.
+     global_loads  1: FP64 D3Q27 node loads through 64-bit global addresses, the path of grids
+                   whose population planes do not fit 32-bit buffer offsets (above 67,108,800
+                   cells).  0 (default): buffer loads where they fit. */
+  int tile_patch;
+  int refill_chunk;
+  int pass1_iters;
+  int global_loads;
+  /* POND_SPLIT with use_mlp: largest predicted change (|du|/sqrt(T), |dT|/T) taken as a
+     start; larger predictions start from the previous gauge.  Default 0.3. */
+  double learned_trust;
 } pond_params;
 
 typedef struct {

@@ -199,6 +199,16 @@ void pond_params_default(pond_params* p, int lattice) {
   // residual; 1e-7 converges reliably above it
   p->split_tol_rel = 1e-7;
   p->split_tol_abs = 1e-12;
+  // Row patches, 384^3 shock-bubble D3Q27 WENO5C: split step 1124 ms in row order, 1025 ms with
+  // 4x4 patches, 1063-1075 ms with 8, 16 or 32; the FP64 step (pair planes, lane refill) 4.10 s
+  // in row order, 3.13 s with 4x4 patches, 3.14 s with 8x8.
+  p->tile_patch = 4;
+  p->refill_chunk = -1;
+  p->pass1_iters = -1;
+  p->global_loads = 0;
+  // 384^3 shock-bubble, learned path: trust radius 0.1 1.290 mean iterations, 307 ms step
+  // kernel; 0.3 and 1.0 1.284, 300 ms
+  p->learned_trust = 0.3;
 }
 
 int pond_qn(int lattice) { return qn_of(lattice); }
@@ -251,23 +261,15 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
   g.nx = (int)p->nx; g.ny = (int)p->ny; g.nz = (int)p->nz;
   g.n = p->nx * p->ny * p->nz;
   g.plane = round_up(g.n, 64);
-  // wave tile of the one-lane-per-cell step kernels: POND_WAVE_TILE="TXxTYxTZ" (product 64),
-  // 64x1x1 by default and on a line
-  g.tx = 64; g.ty = 1; g.tz = 1;
-  if (const char* wt = std::getenv("POND_WAVE_TILE")) {
-    int a = 0, b = 0, c = 0;
-    if (std::sscanf(wt, "%dx%dx%d", &a, &b, &c) != 3 || a < 1 || b < 1 || c < 1 || a * b * c != 64)
-      return bail(POND_ERR_INVALID, "pond_create: POND_WAVE_TILE must be TXxTYxTZ with TX*TY*TZ = 64");
-    if (g.ny > 1 || g.nz > 1) { g.tx = a; g.ty = b; g.tz = c; }
-  }
-  if (g.tx == 64) {
-    g.ntx = (int)((g.n + 63) / 64);
-    g.nty = g.ntz = 1;
-  } else {
-    g.ntx = (g.nx + g.tx - 1) / g.tx;
-    g.nty = (g.ny + g.ty - 1) / g.ty;
-    g.ntz = (g.nz + g.tz - 1) / g.tz;
-  }
+  if (p->tile_patch < 0) return bail(POND_ERR_INVALID, "pond_create: tile_patch must be >= 0");
+  if (p->refill_chunk < -1) return bail(POND_ERR_INVALID, "pond_create: refill_chunk must be >= -1");
+  if (p->pass1_iters < -1 || p->pass1_iters == 0)
+    return bail(POND_ERR_INVALID, "pond_create: pass1_iters must be -1 or >= 1");
+  if (!(p->learned_trust >= 0)) return bail(POND_ERR_INVALID, "pond_create: learned_trust must be >= 0");
+  // row patches where the grid divides into them (ignored elsewhere)
+  const int rp = p->tile_patch;
+  g.rp = (rp > 0 && g.nx % 64 == 0 && g.ny % rp == 0 && g.nz % rp == 0) ? rp : 0;
+  g.ntiles = g.rp > 0 ? (int64_t)(g.nx / 64) * g.ny * g.nz : (g.n + 63) / 64;
   // the split step addresses nine FP32 population planes through one 32-bit buffer offset
   // the split step addresses two population quad planes (16 bytes per cell each) through one
   // 32-bit buffer offset
@@ -282,32 +284,18 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
   a.sqrt_tbase = std::sqrt(ctx->L.TLat);
   a.tlat = ctx->L.TLat;
   a.inv_dim = 1. / ctx->L.Dim;
-  // tuning switches: POND_XCD=0 launch-order workgroups, POND_NO_BUF=1 global loads (FP64)
-  g.xcd = 1;
-  if (const char* v = std::getenv("POND_XCD")) g.xcd = std::atoi(v) != 0;
-  // Row patches of the linear tile order (POND_ROW_PATCH=edge overrides; ignored where the grid
-  // does not divide into them).  Measured on the shock-bubble D3Q27 WENO5C at 384^3: split step
-  // 1124 ms in row order, 1025 ms with 4x4 patches, 1063-1075 ms with 8, 16 or 32; the FP64
-  // step (pair planes, lane refill) 4.10 s in row order, 3.13 s with 4x4 patches, 3.14 s with 8x8.
-  int rp = 4;
-  if (const char* v = std::getenv("POND_ROW_PATCH")) rp = std::atoi(v);
-  g.rp = (rp > 0 && g.tx == 64 && g.nx % 64 == 0 && g.ny % rp == 0 && g.nz % rp == 0) ? rp : 0;
   a.g = g;
-  a.no_buf = 0;
-  if (const char* v = std::getenv("POND_NO_BUF")) a.no_buf = std::atoi(v) != 0;
-  // lane refill in the first pass of the FP64 D3Q27 step (POND_REFILL_CHUNK_F64 overrides, 0 =
-  // one tile per wave).  384^3 shock-bubble FP64 step: one tile per wave 8.07 s, chunks of 2
-  // tiles 6.38 s, 4 tiles 7.05 s (4 tiles in a single pass 6.90 s); with 4x4 row patches
-  // 2 tiles 2.94 s, 3 3.06 s, 6 3.09 s.
-  a.refill_chunk = 2;
-  if (const char* v = std::getenv("POND_REFILL_CHUNK_F64")) a.refill_chunk = std::atoi(v) > 0 ? std::atoi(v) : 0;
+  a.global_loads = p->global_loads != 0;
+  // lane refill in the first pass of the FP64 D3Q27 step.  384^3 shock-bubble FP64 step: one
+  // tile per wave 8.07 s, chunks of 2 tiles 6.38 s, 4 tiles 7.05 s (4 tiles in a single pass
+  // 6.90 s); with 4x4 row patches 2 tiles 2.94 s, 3 3.06 s, 6 3.09 s.
+  a.refill_chunk = p->refill_chunk >= 0 ? p->refill_chunk : 2;
   // Iterations of the first pass of the two-pass step; cells still unconverged are finished by
-  // the second pass (POND_PASS1_ITERS overrides).  Measured on the shock-bubble: FP64 D3Q27,
-  // 128^3: K=3 2.98, K=5 3.19, K=8 3.35, single pass 2.86 MLUPS; split D3Q27, 192^3: K=3 7.6,
-  // K=8 8.9, single pass 11.5 MLUPS (its second pass runs alone after the first and a few
-  // slowly converging cells make a serial tail), so the split step runs in one pass.
-  a.pass1_iters = p->precision == POND_SPLIT ? (1 << 30) : 8;
-  if (const char* k = std::getenv("POND_PASS1_ITERS")) a.pass1_iters = std::atoi(k) > 0 ? std::atoi(k) : a.pass1_iters;
+  // the second pass.  Measured on the shock-bubble: FP64 D3Q27, 128^3: K=3 2.98, K=5 3.19, K=8
+  // 3.35, single pass 2.86 MLUPS; split D3Q27, 192^3: K=3 7.6, K=8 8.9, single pass 11.5 MLUPS
+  // (its second pass runs alone after the first and a few slowly converging cells make a serial
+  // tail), so the split step runs in one pass.
+  a.pass1_iters = p->pass1_iters > 0 ? p->pass1_iters : p->precision == POND_SPLIT ? (1 << 30) : 8;
   for (int i = 0; i < ctx->Qn; i++) a.w[i] = ctx->L.w[i];
 
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
@@ -348,31 +336,23 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
     }
     SplitArgs& sa = ctx->sargs;
     sa.pass1_iters = a.pass1_iters;
-    // lane refill over chunks of 64-cell tiles (POND_REFILL_CHUNK overrides, 0 = one tile per
-    // wave).  Measured on the 384^3 shock-bubble, split step ms, with paired x-slice loads: one
-    // tile 1065, chunks of 2 tiles 805, 3 976, 4 775-778, 6 763, 8 840, 12 910 (one slice per
-    // load and one tile per wave: 1025).  Larger chunks keep more lanes busy but spread the
-    // active cells of a wave over more rows, so its loads touch more cache lines.  With the
-    // current node loop (gather-bound): 3 319, 4 293, 5 305, 6 279, 7 321, 8 317, 9 328, 12 321.
-    // Six tiles are one whole x row at 384^3: with row patches the tiles are 64-cell row
-    // segments, nx/64 per row, so chunks of whole rows let the waves resident on an XCD cover
-    // a compact patch of rows (traffic from beyond L2 per launch: 1.23 TB with chunks of 4
-    // tiles, 0.50 TB with 6; at 256^3, 4 tiles per row: chunks of 4 113 ms, 3 128, 6 127, 8 124).
-    // Default: whole rows, at least 4 tiles.
+    // lane refill over chunks of 64-cell tiles.  Measured on the 384^3 shock-bubble, split step
+    // ms, with paired x-slice loads: one tile 1065, chunks of 2 tiles 805, 3 976, 4 775-778, 6
+    // 763, 8 840, 12 910 (one slice per load and one tile per wave: 1025).  Larger chunks keep
+    // more lanes busy but spread the active cells of a wave over more rows, so its loads touch
+    // more cache lines.  With the current node loop (gather-bound): 3 319, 4 293, 5 305, 6 279,
+    // 7 321, 8 317, 9 328, 12 321.  Six tiles are one whole x row at 384^3: with row patches the
+    // tiles are 64-cell row segments, nx/64 per row, so chunks of whole rows let the waves
+    // resident on an XCD cover a compact patch of rows (traffic from beyond L2 per launch:
+    // 1.23 TB with chunks of 4 tiles, 0.50 TB with 6; at 256^3, 4 tiles per row: chunks of 4
+    // 113 ms, 3 128, 6 127, 8 124).  Default: whole rows, at least 4 tiles.
     sa.refill_chunk = 4;
     if (g.rp > 0) {
       const int wpr = g.nx / 64;
       sa.refill_chunk = wpr * ((4 + wpr - 1) / wpr);
     }
-    if (const char* v = std::getenv("POND_REFILL_CHUNK")) sa.refill_chunk = std::atoi(v) > 0 ? std::atoi(v) : 0;
-    // largest predicted change taken as a start (POND_LEARNED_TRUST overrides).  384^3
-    // shock-bubble, learned path: radius 0.1 1.290 mean iterations, 307 ms step kernel; 0.3 and
-    // 1.0 1.284, 300 ms; the error against FP64 outside the influence of capped cells is the
-    // same (3.1e-6 on u) for all three
-    sa.learned_trust = 0.3f;
-    if (const char* v = std::getenv("POND_LEARNED_TRUST")) sa.learned_trust = (float)std::atof(v);
-    sa.weno5c_variant = 0;
-    if (const char* v = std::getenv("POND_SPLIT_VARIANT")) sa.weno5c_variant = std::atoi(v);
+    if (p->refill_chunk >= 0) sa.refill_chunk = p->refill_chunk;
+    sa.learned_trust = (float)p->learned_trust;
     sa.g = g;
     sa.dtau = p->dtau;
     sa.tol_abs = p->split_tol_abs;

@@ -12,15 +12,12 @@ struct Grid {
   int nx, ny, nz;
   int64_t n;
   int64_t plane;
-  // 64-cell wave tile of the one-lane-per-cell kernels (tx*ty*tz = 64) and the number of tiles
-  // along each axis; the tiles cover the grid, lanes outside it are idle.  tx = 64: linear
-  // order, ntx*nty*ntz = ceil(n/64) waves
-  int tx, ty, tz;
-  int ntx, nty, ntz;
-  int xcd;  // 1: XCD-contiguous workgroup order (xcd_block), 0: launch order
-  // linear tiles only: rows (y, z) taken in rp x rp patches (y fastest within a patch, patches
-  // y fastest then z) instead of y fastest over the whole plane; 0 = plain row order.  Needs
-  // nx % 64 == 0, ny % rp == 0 and nz % rp == 0.
+  // 64-cell wave tiles of the one-lane-per-cell kernels: tile b holds 64 cells of one x row
+  // (rp > 0) or the cells 64b .. 64b+63 of the linear order, across row ends (rp = 0)
+  int64_t ntiles;
+  // rows (y, z) taken in rp x rp patches (y fastest within a patch, patches y fastest then z)
+  // instead of y fastest over the whole plane; 0 = plain row order.  Needs nx % 64 == 0,
+  // ny % rp == 0 and nz % rp == 0.
   int rp;
 };
 
@@ -34,7 +31,7 @@ struct StepArgs {
   double tlat;
   double inv_dim;     // 1./Dim (structs.cuh:330)
   int pass1_iters;    // iterations of the first pass of the one-lane-per-cell step
-  int no_buf;         // 1: global loads even where the buffer-resource loads apply
+  int global_loads;   // 1: global loads even where the buffer-resource loads apply
   int refill_chunk;   // D3Q27 first pass: 64-cell tiles per wave with lane refill (0: one tile per wave)
   double w[kQmax];    // lattice weights for the collide and set_base
 };
@@ -100,16 +97,15 @@ __device__ __forceinline__ int floor_int(double v) {
 // workgroups resident on one XCD overlap in its L2.  A bijection on [0, nblocks) for any count.
 __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks) {
   constexpr int kXcd = 8;
-  if (nblocks < 0) return b;
   const int64_t q = nblocks / kXcd, r = nblocks % kXcd;
   const int64_t x = b % kXcd, local = b / kXcd;
   return x * q + (x < r ? x : r) + local;
 }
 
-// cell (x, y, z) of lane `lane` in tile `b` (tiles x fastest); false outside the grid.  The
-// 64x1x1 tile is the linear order: wave b holds cells 64b .. 64b+63, across row ends.
+// cell (x, y, z) of lane `lane` in tile `b`; false outside the grid.  Without row patches the
+// tiles are the linear order: tile b holds cells 64b .. 64b+63, across row ends.
 __device__ __forceinline__ bool tile_cell(const Grid& g, int64_t b, int lane, int& x, int& y, int& z) {
-  if (g.tx == 64 && g.rp > 0) {
+  if (g.rp > 0) {
     // the workgroups resident on one XCD at a time then cover a compact (y, z) patch of rows,
     // whose stencil halo is smaller than that of a run of rows along y
     const int wpr = g.nx / 64;
@@ -121,19 +117,11 @@ __device__ __forceinline__ bool tile_cell(const Grid& g, int64_t b, int lane, in
     z = (int)(t / npy) * g.rp + w / g.rp;
     return row < (int64_t)g.ny * g.nz;
   }
-  if (g.tx == 64) {
-    const int64_t c = b * 64 + lane;
-    x = (int)(c % g.nx);
-    y = (int)((c / g.nx) % g.ny);
-    z = (int)(c / ((int64_t)g.nx * g.ny));
-    return c < g.n;
-  }
-  const int lx = lane % g.tx, ly = (lane / g.tx) % g.ty, lz = lane / (g.tx * g.ty);
-  const int64_t bx = b % g.ntx, r = b / g.ntx;
-  x = (int)bx * g.tx + lx;
-  y = (int)(r % g.nty) * g.ty + ly;
-  z = (int)(r / g.nty) * g.tz + lz;
-  return x < g.nx && y < g.ny && z < g.nz;
+  const int64_t c = b * 64 + lane;
+  x = (int)(c % g.nx);
+  y = (int)((c / g.nx) % g.ny);
+  z = (int)(c / ((int64_t)g.nx * g.ny));
+  return c < g.n;
 }
 
 // FP64 population storage.  D3Q27 keeps the populations in fourteen pair planes: population k

@@ -137,10 +137,11 @@ __device__ __forceinline__ bool node_ok(const NodeBuf& s, uint32_t boff) {
 
 // D3Q27: the node's gauge record (ux, uy, uz, sqrt(|T|)*sign(T)), 32 bytes per cell, written by
 // k_node_prep into the P buffer, at byte offset boff of an FP64 plane (boff = 8 * cell)
+// (64-bit byte offsets: 4 * boff leaves 32 bits from 2^27 cells on, the grids this path serves)
 __device__ __forceinline__ void node_g4(const NodeSrc& s, uint32_t boff, double& ux, double& uy, double& uz,
                                        double& P) {
-  const double2 a = *(const double2*)((const char*)s.P + 4u * boff);
-  const double2 b = *(const double2*)((const char*)s.P + 4u * boff + 16u);
+  const double2 a = *(const double2*)((const char*)s.P + 4ull * boff);
+  const double2 b = *(const double2*)((const char*)s.P + 4ull * boff + 16ull);
   ux = a.x; uy = a.y; uz = b.x; P = b.y;
 }
 __device__ __forceinline__ double dbl_of(uint32_t lo, uint32_t hi) {
@@ -160,7 +161,7 @@ __device__ __forceinline__ void node_g4(const NodeBuf& s, uint32_t boff, double&
 __device__ __forceinline__ void node_f27(const NodeSrc& s, uint32_t boff, double f[27]) {
 #pragma unroll
   for (int j = 0; j < 14; j++) {
-    const double2 v = *(const double2*)((const char*)(s.f + (int64_t)2 * j * s.g.plane) + 2u * boff);
+    const double2 v = *(const double2*)((const char*)(s.f + (int64_t)2 * j * s.g.plane) + 2ull * boff);
     f[2 * j] = v.x;
     if (2 * j + 1 < 27) f[2 * j + 1] = v.y;
   }
@@ -595,7 +596,7 @@ __global__ __launch_bounds__(64) void k_step_f64(const NodeSrc gsrc, double* __r
   else src = gsrc;
   const Grid& g = a.g;
   int x, y, z;
-  const bool active = tile_cell(g, xcd_block(blockIdx.x, g.xcd ? (int64_t)gridDim.x : -1), threadIdx.x, x, y, z);
+  const bool active = tile_cell(g, xcd_block(blockIdx.x, gridDim.x), threadIdx.x, x, y, z);
   const int64_t cell = active ? cell_index(g, x, y, z) : 0;
   bool pending = false;
   F64Iter st;
@@ -630,8 +631,8 @@ __global__ __launch_bounds__(64) void k_step_f64_refill(const NodeSrc gsrc, doub
   else src = gsrc;
   const Grid& g = a.g;
   const int lane = threadIdx.x;
-  const int64_t ntiles = (int64_t)g.ntx * g.nty * g.ntz;
-  const int64_t t0 = xcd_block(blockIdx.x, g.xcd ? (int64_t)gridDim.x : -1) * chunk;
+  const int64_t ntiles = g.ntiles;
+  const int64_t t0 = xcd_block(blockIdx.x, gridDim.x) * chunk;
   const int64_t total = (ntiles - t0 < chunk ? ntiles - t0 : chunk) * 64;  // cell slots of the chunk
   const int stop = a.pass1_iters < a.max_iter ? a.pass1_iters : a.max_iter;
   int64_t next = 0;  // first unstarted slot (the same in every lane)
@@ -719,7 +720,7 @@ __global__ __launch_bounds__(128) void k_step_q125_f64(const double* __restrict_
   __shared__ int sok[27];        // node populations finite and below 2^200
   const Grid& g = a.g;
   const int64_t P = g.plane;
-  const int64_t cell = xcd_block(blockIdx.x, g.xcd ? (int64_t)gridDim.x : -1);
+  const int64_t cell = xcd_block(blockIdx.x, gridDim.x);
   const int k = threadIdx.x;
   const bool lane_ok = k < Q;
   const int x = (int)(cell % g.nx);
@@ -913,12 +914,12 @@ hipError_t launch_step_f64(int lattice, int interp, const double* fin, const dou
                        a);
     return hipGetLastError();
   }
-  const unsigned blocks = (unsigned)((int64_t)a.g.ntx * a.g.nty * a.g.ntz);
+  const unsigned blocks = (unsigned)(a.g.ntiles);
   const NodeSrc src{fin, min, Pin, okin, a.g};
   // buffer-resource loads while nine FP64 planes (D1Q5) or four pair planes (D3Q27) fit a 32-bit
   // offset
   const bool buf = (uint64_t)a.g.plane * 8u * (lattice == kD3Q27 ? 8u : (uint64_t)kPlanesPerRsrc64) <= 0xffffffffull &&
-                   !a.no_buf;
+                   !a.global_loads;
   hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, st, pend.count);
   const int64_t w2 = (a.g.n + 63) / 64;
   const unsigned blocks2 = (unsigned)(w2 < 16384 ? w2 : 16384);

@@ -14,8 +14,6 @@ struct SplitArgs {
   double tlat;
   double inv_dim;
   int pass1_iters;  // iterations of the first pass of the D3Q27 step (the rest go to the second)
-  int weno5c_variant;   // D3Q27 WENO5C kernel: 0 grouped targets (lane refill), 2 per target, 3 grouped
-                        // with one x slice per trip (one tile per wave)
   float learned_trust;  // largest predicted change (thermal units) taken as a start
   int refill_chunk;     // D3Q27 WENO5C single pass: 64-cell tiles per wave with lane refill (0: one tile per wave)
   float w[kQmax];

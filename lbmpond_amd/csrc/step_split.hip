@@ -295,8 +295,7 @@ __device__ float interp_lagr4f(const NodeSrcF& src, double cx, double cy, double
 // transfers load the node three times and contract it three times.  The x slices of the three
 // five-slice WENO windows are visited once each (their union, at most 5 + |r(-1) - r(+1)|).
 // Same interpolation as interp_weno5f<true> per target, with the y/z weights summed in a
-// different order.
-template <bool PAIRS>
+// different order.  The x slices are visited two at a time (more loads in flight per lane).
 __device__ void interp_group_weno5c(const NodeSrcF& src, int x, int y, int z, double s, const double gc[4], int ey,
                                     int ez, float sqTb, float out[3]) {
   const Grid& g = src.g;
@@ -408,13 +407,13 @@ __device__ void interp_group_weno5c(const NodeSrcF& src, int x, int y, int z, do
       }
     }
   };
-  for (int xs = xlo; xs <= xhi; xs += PAIRS ? 2 : 1) {
+  for (int xs = xlo; xs <= xhi; xs += 2) {
     const int xw = wrapi(xs, g.nx);
     float yz[3] = {0.f, 0.f, 0.f}, mid[3] = {0.f, 0.f, 0.f};
     float yz2[3] = {0.f, 0.f, 0.f}, mid2[3] = {0.f, 0.f, 0.f};
-    // PAIRS: slices xs and xs+1 together (the second through the same per-node code)
-    const bool two = PAIRS && xs + 1 <= xhi;
-    const int xw2 = PAIRS ? wrapi(xs + 1, g.nx) : 0;
+    // slices xs and xs+1 together (the second through the same per-node code)
+    const bool two = xs + 1 <= xhi;
+    const int xw2 = wrapi(xs + 1, g.nx);
     // one node at a time: its loads are the only ones in flight per lane
 #pragma unroll 1
     for (int q = 0; q < 9; q++) {
@@ -562,12 +561,12 @@ __device__ __forceinline__ bool split27_once(const NodeSrcF& src, float* __restr
       kahan_add(bzs, bzc, (float)ez * fn);
       cs += (double)fn * (double)(ex * ex + ey * ey + ez * ez);
     };
-    if constexpr (INTERP == 2 || INTERP == 4) {
+    if constexpr (INTERP == 2) {
       // WENO5C: the three targets of each (ey, ez) group share their node loads and y/z work
       for (int gi = 0; gi < 9; gi++) {
         const int ey = gi / 3 - 1, ez = gi % 3 - 1;
         float f3[3];
-        interp_group_weno5c<INTERP == 2>(src, x, y, z, s, gc, ey, ez, sqTb, f3);
+        interp_group_weno5c(src, x, y, z, s, gc, ey, ez, sqTb, f3);
         add(-1, ey, ez, f3[0]);
         add(0, ey, ez, f3[1]);
         add(1, ey, ez, f3[2]);
@@ -580,9 +579,7 @@ __device__ __forceinline__ bool split27_once(const NodeSrcF& src, float* __restr
         const double vz = s * (double)ez + gc[2];
         const TgtF t = make_tgtf(i, gc);
         const double cx = (double)x - vx, cy = (double)y - vy, cz = (double)z - vz;
-        const float fn = INTERP == 1   ? interp_lagr4f(src, cx, cy, cz, t, sqTb)
-                         : INTERP == 3 ? interp_weno5f<true>(src, cx, cy, cz, t, sqTb)
-                                       : interp_weno5f<false>(src, cx, cy, cz, t, sqTb);
+        const float fn = INTERP == 1 ? interp_lagr4f(src, cx, cy, cz, t, sqTb) : interp_weno5f<false>(src, cx, cy, cz, t, sqTb);
         add(ex, ey, ez, fn);
       }
     }
@@ -717,8 +714,7 @@ __device__ __forceinline__ void split27_start(const float* __restrict__ min, con
 // (wave-aggregated: one atomic per wave) and finished by k_step_split27_rest, whose waves
 // hold only such cells.  Each cell's arithmetic is the same as in a single pass.
 // INTERP: 0 WENO5 as written, 1 Lagrange-4, 2 WENO5C with the targets of an (ey, ez) group
-// sharing their node work and two x slices loaded together, 3 WENO5C target by target, 4 as 2
-// with one x slice at a time.
+// sharing their node work.
 template <int INTERP>
 __device__ __forceinline__ void split27_pass1(const float* __restrict__ fin, const float* __restrict__ min,
                                               const double* __restrict__ Tin, const float* __restrict__ g4in,
@@ -728,7 +724,7 @@ __device__ __forceinline__ void split27_pass1(const float* __restrict__ fin, con
                                               const SplitArgs& a) {
   const Grid& g = a.g;
   int x, y, z;
-  const bool live = tile_cell(g, xcd_block(blockIdx.x, g.xcd ? (int64_t)gridDim.x : -1), threadIdx.x, x, y, z);
+  const bool live = tile_cell(g, xcd_block(blockIdx.x, gridDim.x), threadIdx.x, x, y, z);
   bool pending = false;
   SplitIter st;
   const int64_t cell = live ? cell_index(g, x, y, z) : 0;
@@ -777,8 +773,8 @@ __device__ __forceinline__ void split27_refill(const float* __restrict__ fin, co
                                                const float* __restrict__ guess, const SplitArgs& a, int chunk) {
   const Grid& g = a.g;
   const int lane = threadIdx.x;
-  const int64_t ntiles = (int64_t)g.ntx * g.nty * g.ntz;
-  const int64_t w = xcd_block(blockIdx.x, g.xcd ? (int64_t)gridDim.x : -1);
+  const int64_t ntiles = g.ntiles;
+  const int64_t w = xcd_block(blockIdx.x, gridDim.x);
   const int64_t t0 = w * chunk;
   const int64_t total = (ntiles - t0 < chunk ? ntiles - t0 : chunk) * 64;  // cell slots of the chunk
   const NodeSrcF src = make_src(fin, g4in, g);
@@ -898,7 +894,7 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
   __shared__ double red[10];    // per-wave partial sums of the five moments
   const Grid& g = a.g;
   const int64_t P = g.plane;
-  const int64_t cell = xcd_block(blockIdx.x, g.xcd ? (int64_t)gridDim.x : -1);
+  const int64_t cell = xcd_block(blockIdx.x, gridDim.x);
   const int k = threadIdx.x;
   const bool lane_ok = k < Q;
   const int x = (int)(cell % g.nx);
@@ -1094,7 +1090,7 @@ hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const
     return hipGetLastError();
   }
   if (lattice != kD3Q27) return hipErrorInvalidValue;
-  const unsigned blocks = (unsigned)((int64_t)a.g.ntx * a.g.nty * a.g.ntz);
+  const unsigned blocks = (unsigned)(a.g.ntiles);
   hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, st, pend.count);
   // second-pass grid: enough waves to fill the chip several times, never more than the cells
   const int64_t w2 = (a.g.n + 63) / 64;
@@ -1106,7 +1102,7 @@ hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const
     hipLaunchKernelGGL((k_step_split27_rest<I>), dim3(blocks2), dim3(64), 0, st, in.f, in.m, in.T, g4, out.f,     \
                        out.m, out.T, niter, pend, a);                                                             \
   } while (0)
-  if (interp == 2 && a.weno5c_variant == 0 && a.refill_chunk > 0 && a.pass1_iters >= a.max_iter + kLearnedBudget) {
+  if (interp == 2 && a.refill_chunk > 0 && a.pass1_iters >= a.max_iter + kLearnedBudget) {
     // (measured alternatives at 384^3: one x slice per trip 372 ms, held to 3 waves per SIMD
     // with spills 360-400 ms, against 317 ms for two slices at 2 waves per SIMD)
     const int64_t waves = ((int64_t)blocks + a.refill_chunk - 1) / a.refill_chunk;
@@ -1115,8 +1111,6 @@ hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const
     return hipGetLastError();
   }
   if (interp == 1) POND_LAUNCH_S(k_step_split27<1>, 1);
-  else if (interp == 2 && a.weno5c_variant == 2) POND_LAUNCH_S(k_step_split27<3>, 3);
-  else if (interp == 2 && a.weno5c_variant == 3) POND_LAUNCH_S(k_step_split27<4>, 4);
   else if (interp == 2) POND_LAUNCH_S(k_step_split27<2>, 2);
   else POND_LAUNCH_S(k_step_split27<0>, 0);
 #undef POND_LAUNCH_S

@@ -54,6 +54,11 @@ class PondParams(ctypes.Structure):
         ("use_mlp", ctypes.c_int),
         ("split_tol_rel", ctypes.c_double),
         ("split_tol_abs", ctypes.c_double),
+        ("tile_patch", ctypes.c_int),
+        ("refill_chunk", ctypes.c_int),
+        ("pass1_iters", ctypes.c_int),
+        ("global_loads", ctypes.c_int),
+        ("learned_trust", ctypes.c_double),
     ]
 
 
@@ -181,7 +186,11 @@ class Pond:
 
     def __init__(self, lattice="D3Q27", shape=(16, 16, 16), interp="weno5", precision="fp64", dtau=None,
                  tol_abs=None, tol_rel=None, max_iter=None, use_mlp=False, split_tol_rel=None,
-                 split_tol_abs=None):
+                 split_tol_abs=None, tile_patch=None, refill_chunk=None, pass1_iters=None, global_loads=None,
+                 learned_trust=None):
+        """Parameters left at None keep pond_params_default's value.  tile_patch, refill_chunk,
+        pass1_iters and global_loads set the D3Q27 kernels' work distribution (no effect on
+        results); learned_trust the learned start's trust radius (include/pond.h)."""
         L = lib()
         p = PondParams()
         L.pond_params_default(ctypes.byref(p), LATTICES[lattice])
@@ -200,6 +209,12 @@ class Pond:
             p.split_tol_rel = float(split_tol_rel)
         if split_tol_abs is not None:
             p.split_tol_abs = float(split_tol_abs)
+        for name, val in (("tile_patch", tile_patch), ("refill_chunk", refill_chunk), ("pass1_iters", pass1_iters),
+                          ("global_loads", global_loads)):
+            if val is not None:
+                setattr(p, name, int(val))
+        if learned_trust is not None:
+            p.learned_trust = float(learned_trust)
         p.use_mlp = 1 if use_mlp else 0
         h = ctypes.c_void_p()
         st = L.pond_create(ctypes.byref(p), ctypes.byref(h))

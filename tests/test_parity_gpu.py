@@ -186,7 +186,7 @@ def test_nonfinite_and_absurd_gauges_stay_in_bounds(gpu):
             assert np.array_equal(got[k][fin], ref[k][fin]), k
 
 
-def test_two_pass_fp64_is_independent_of_first_pass_length(gpu, monkeypatch):
+def test_two_pass_fp64_is_independent_of_first_pass_length(gpu):
     """The FP64 step's second pass resumes cells bit for bit: any first-pass length gives the
     oracle's result."""
     shape = (16, 12, 10)
@@ -195,9 +195,8 @@ def test_two_pass_fp64_is_independent_of_first_pass_length(gpu, monkeypatch):
     o.set_state(*f)
     o.step(3)
     ref = o.fields()
-    for k in ("1", "2"):
-        monkeypatch.setenv("POND_PASS1_ITERS", k)
-        with Pond("D3Q27", shape, "weno5c", "fp64") as p:
+    for k in (1, 2):
+        with Pond("D3Q27", shape, "weno5c", "fp64", pass1_iters=k) as p:
             p.set_state(*f)
             p.step(3)
             got = p.fields()
@@ -206,7 +205,7 @@ def test_two_pass_fp64_is_independent_of_first_pass_length(gpu, monkeypatch):
 
 
 @pytest.mark.parametrize("interp", ["weno5", "lagr4", "weno5c"])
-def test_fp64_lane_refill_matches_oracle(gpu, monkeypatch, interp):
+def test_fp64_lane_refill_matches_oracle(gpu, interp):
     """The FP64 first pass with lane refill (a wave owns several 64-cell tiles and its lanes take
     new cells as theirs finish), with and without a second pass, gives the oracle's result bit for
     bit, on a grid whose last tile is ragged."""
@@ -216,11 +215,9 @@ def test_fp64_lane_refill_matches_oracle(gpu, monkeypatch, interp):
     o.set_state(*f)
     o.step(3)
     ref = o.fields()
-    for chunk, k1 in (("1", "8"), ("3", "2"), ("8", "1000")):
-        monkeypatch.setenv("POND_REFILL_CHUNK_F64", chunk)
-        monkeypatch.setenv("POND_PASS1_ITERS", k1)
+    for chunk, k1 in ((1, 8), (3, 2), (8, 1000)):
         print(f"[refill f64] {interp} chunk {chunk} first pass {k1}", flush=True)
-        with Pond("D3Q27", shape, interp, "fp64") as p:
+        with Pond("D3Q27", shape, interp, "fp64", refill_chunk=chunk, pass1_iters=k1) as p:
             p.set_state(*f)
             p.step(3)
             got = p.fields()

@@ -138,34 +138,29 @@ def test_split_d3q125_turbulence(gpu):
     assert got["niter_hist"].mean() <= ref["niter_hist"].mean()
 
 
-def test_two_pass_split_is_independent_of_first_pass_length(gpu, monkeypatch):
+def test_two_pass_split_is_independent_of_first_pass_length(gpu):
     """Cells left unconverged by the first pass resume in the second with the same state: the
-    result is bitwise the same for any first-pass length (POND_PASS1_ITERS)."""
+    result is bitwise the same for any first-pass length (pond_params.pass1_iters)."""
     case = ic.shock_bubble(n=24)
     outs = []
-    for k in ("1", "3", "100"):
-        monkeypatch.setenv("POND_PASS1_ITERS", k)
-        outs.append(run("D3Q27", case.shape, "weno5c", "split", case.fields, 3, dtau=case.dtau))
+    for k in (1, 3, 100):
+        outs.append(run("D3Q27", case.shape, "weno5c", "split", case.fields, 3, dtau=case.dtau, pass1_iters=k))
     for o in outs[:2]:
         for key in ("rho", "ux", "uy", "uz", "T", "niter"):
             assert np.array_equal(o[key], outs[2][key]), key
 
 
-def test_refill_and_paired_loads_match_plain_kernel(gpu, monkeypatch):
+def test_refill_matches_one_tile_per_wave(gpu):
     """The lane-refill single pass (a wave owns several 64-cell tiles and its lanes take new cells
-    as theirs converge) and the paired x-slice loads (two neighbouring cells' planes in one 8-byte
-    load) give bitwise the results of one tile per wave with one slice per load, on a
-    shock-bubble state and on a grid whose rows are shorter than a wave and whose last tile is
-    ragged (row ends inside a wave, where the pairs fall back to single loads)."""
+    as theirs converge) gives bitwise the results of one tile per wave, on a shock-bubble state
+    and on a grid whose rows are shorter than a wave and whose last tile is ragged."""
     cases = ((ic.shock_bubble(n=40).fields, (40, 40, 40), 4),
              (smooth_random_state((37, 20, 11), seed=71, T0=0.01), (37, 20, 11), 3))
     for fields, shape, steps in cases:
         outs = []
-        for variant, chunk in (("3", "0"), ("0", "0"), ("0", "1"), ("0", "3"), ("0", "8")):
-            monkeypatch.setenv("POND_SPLIT_VARIANT", variant)
-            monkeypatch.setenv("POND_REFILL_CHUNK", chunk)
-            print(f"[refill] {shape} variant {variant} chunk {chunk}", flush=True)
-            with Pond("D3Q27", shape, "weno5c", "split") as p:
+        for chunk in (0, 1, 3, 8):
+            print(f"[refill] {shape} chunk {chunk}", flush=True)
+            with Pond("D3Q27", shape, "weno5c", "split", refill_chunk=chunk) as p:
                 p.set_state(*fields)
                 p.step(steps)
                 outs.append(p.fields())
