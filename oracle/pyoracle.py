@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "libpond_oracle.so")
+# POND_ORACLE_LIB names another build of the oracle (oracle/mutation_check.py runs the pin
+# tests against deliberately broken builds that way)
+LIB_PATH = os.environ.get("POND_ORACLE_LIB") or os.path.join(_HERE, "_build", "libpond_oracle.so")
 LATTICES = {"D1Q5": 0, "D3Q27": 1, "D3Q125": 2}
 INTERPS = {"weno5": 0, "lagr4": 1, "weno5c": 2}
 QN = {"D1Q5": 5, "D3Q27": 27, "D3Q125": 125}

@@ -364,3 +364,161 @@ def test_acoustic_wave_speed_gamma3():
         speed = -np.angle(mode) / k / steps
         assert abs(speed - c) < 2e-4 * c, (interp, speed, c)
         assert abs(np.abs(mode) * 2 / n / eps - 1) < 1e-3
+
+
+# ----------------------------------------------------------------------------- WENO5 weights
+
+def weno5_numpy(u, c):
+    """Jiang & Shu (J. Comput. Phys. 126, 1996) WENO5 point value from the five node values
+    u[0..4] at x = r-2 .. r+2, evaluated at offset c = x - r: three quadratic sub-stencil
+    interpolants, linear weights d_k that reproduce the quartic through all five nodes, and
+    smoothness indicators in the difference form beta = 13/12 (second difference)^2 +
+    1/4 (one-sided first difference)^2, weights d_k / (1e-6 + beta_k)^2."""
+    def quad(v0, v1, v2, s):  # Lagrange quadratic on nodes 0, 1, 2 at s
+        return v0 * (s - 1) * (s - 2) / 2 - v1 * s * (s - 2) + v2 * s * (s - 1) / 2
+    S = [quad(u[k], u[k + 1], u[k + 2], c + 2 - k) for k in range(3)]
+    # linear weights: the unique d_k with sum_k d_k S_k = the quartic interpolant for any data
+    d0 = (c - 1) * (c - 2) / 12
+    d2 = (c + 1) * (c + 2) / 12
+    d = [d0, 1 - d0 - d2, d2]
+    u1, u2, u3, u4, u5 = u
+    b = [13 / 12 * (u1 - 2 * u2 + u3) ** 2 + 0.25 * (u1 - 4 * u2 + 3 * u3) ** 2,
+         13 / 12 * (u2 - 2 * u3 + u4) ** 2 + 0.25 * (u2 - u4) ** 2,
+         13 / 12 * (u3 - 2 * u4 + u5) ** 2 + 0.25 * (3 * u3 - 4 * u4 + u5) ** 2]
+    w = [d[k] / (1e-6 + b[k]) ** 2 for k in range(3)]
+    return sum(w[k] * S[k] for k in range(3)) / sum(w)
+
+
+@pytest.mark.parametrize("interp", ["weno5", "weno5c"])
+def test_weno5_nonlinear_weights_match_jiang_shu(interp):
+    """A field varying only along x, with a steep front, so the three smoothness indicators
+    differ by orders of magnitude and the nonlinear weights are far from the linear ones: the
+    oracle's interpolation equals an independent WENO5 with Jiang-Shu difference-form
+    indicators.  A wrong normalisation of any indicator, a swapped pair or wrong linear weights
+    move the result by far more than the tolerance."""
+    shape = (24, 6, 5)
+    nx, ny, nz = shape
+    gauge = np.array([0.0, 0.0, 0.0, 0.3])
+    z, y, x = [a.reshape(-1).astype(float) for a in np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx),
+                                                                   indexing="ij")]
+
+    def F(xx):
+        return 1.0 + 0.4 * np.tanh((xx - 11.3) / 0.8) + 0.05 * np.sin(0.7 * xx)
+
+    f, mom = uniform_gauge_state("D3Q27", shape, F(x), gauge)
+    o = po.Oracle("D3Q27", shape, interp)
+    o.set_populations(f, *mom)
+    rng = np.random.default_rng(8)
+    spread = []
+    for _ in range(24):
+        crd = np.array([rng.uniform(8.6, 14.4), rng.uniform(1.6, 3.4), rng.uniform(1.6, 2.4)])
+        r = int(math.floor(crd[0] + 0.5))
+        u = F(np.arange(r - 2, r + 3, dtype=float))
+        exact = weno5_numpy(u, crd[0] - r)
+        got = o.interp(5, crd, gauge)
+        assert abs(got - exact) <= 1e-12 * max(1.0, abs(exact)), (crd, got, exact)
+        # how far the nonlinear weights move the value from the linear (quartic) one
+        c = crd[0] - r
+        spread.append(abs(exact - np.polyval(np.polyfit(np.arange(-2, 3), u, 4), c)))
+    assert max(spread) > 1e-3  # the test exercises strongly nonlinear weights
+
+
+@pytest.mark.parametrize("interp,left", [("weno5", 0), ("weno5c", -1)])
+def test_weno5_yz_substencil_nodes_on_a_cubic(interp, left):
+    """F = 1 + a (y - y0)^3 + b (z - z0)^2, flat in x: every smoothness indicator is zero (so the
+    weights are the linear ones, which sum to one) and the interpolant is the y/z quadratic
+    through the sub-stencil's nodes.  Quadratic interpolation of a cubic on nodes n, n+1, n+2
+    misses it by exactly a (s - n)(s - n - 1)(s - n - 2), so the value fixes which nodes were
+    used: floor(y + 1/2) + 0..2 as written (steps.cu:776-778), floor(y + 1/2) - 1..+1 centred."""
+    shape = (10, 24, 24)
+    nx, ny, nz = shape
+    a, b, y0, z0 = 0.002, 0.003, 11.0, 12.0
+    gauge = np.array([0.0, 0.0, 0.0, 0.3])
+    z, y, x = [v.reshape(-1).astype(float) for v in np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx),
+                                                                   indexing="ij")]
+
+    def F(yy, zz):
+        return 1 + a * (yy - y0) ** 3 + b * (zz - z0) ** 2
+
+    f, mom = uniform_gauge_state("D3Q27", shape, F(y, z), gauge)
+    o = po.Oracle("D3Q27", shape, interp)
+    o.set_populations(f, *mom)
+    rng = np.random.default_rng(9)
+    for _ in range(12):
+        crd = np.array([rng.uniform(3, 6), rng.uniform(9.0, 13.0), rng.uniform(10.0, 14.0)])
+        n = math.floor(crd[1] + 0.5) + left
+        s = crd[1]
+        exact = F(crd[1], crd[2]) - a * (s - n) * (s - n - 1) * (s - n - 2)
+        got = o.interp(11, crd, gauge)
+        assert abs(got - exact) < 1e-12, (crd, got, exact)
+
+
+# ----------------------------------------------------------------------------- composed steps in 3D
+
+def acoustic_wave(lat, interp, shape, axis, steps, sign=1, T0=0.05, eps=1e-4):
+    """a small-amplitude sound wave along `axis` (gamma = 5/3 in 3D): relative phase speed and
+    amplitude of its density mode after `steps` steps"""
+    g = 5.0 / 3.0
+    n = int(np.prod(shape))
+    nx, ny, nz = shape
+    z, y, x = [v.reshape(-1).astype(float) for v in np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx),
+                                                                   indexing="ij")]
+    X = (x, y, z)[axis]
+    k = 2 * np.pi / shape[axis]
+    c = math.sqrt(g * T0)
+    ph = np.cos(k * X)
+    rho = 1 + eps * ph
+    p = T0 * (1 + g * eps * ph)
+    u = [np.zeros(n) for _ in range(3)]
+    u[axis] = sign * c * eps * ph
+    o = po.Oracle(lat, shape, interp)
+    o.set_state(rho, u[0], u[1], u[2], p / rho)
+    o.step(steps)
+    d = o.fields()["rho"] - 1
+    mode = np.sum(d * np.exp(-1j * k * X))
+    return -np.angle(mode) / k / steps / (sign * c), np.abs(mode) * 2 / n / eps
+
+
+@pytest.mark.parametrize("lat,interp,edge", [("D3Q125", "weno5", 16), ("D3Q27", "lagr4", 16),
+                                             ("D3Q27", "weno5c", 16)])
+def test_acoustic_waves_3d(lat, interp, edge):
+    """Sound waves along x, y and z, running both ways: each travels at sqrt(gamma T) with
+    gamma = 5/3 (to the scheme's dispersion error at 12-16 cells per wavelength) without
+    growing, and the step treats the three axes alike (the tensor-product schemes to round-off:
+    D3Q125 and Lagrange-4 are the same along every axis; WENO5C is WENO along x and quadratic
+    along y and z, so y and z agree).  A departure point taken at x + v, a stencil origin off
+    along one axis, or a transposed axis fails this."""
+    speeds = {}
+    for axis in range(3):
+        shape = [3, 3, 3]
+        shape[axis] = edge
+        for sign in (1, -1):
+            v, amp = acoustic_wave(lat, interp, tuple(shape), axis, 8, sign)
+            print(lat, interp, "axis", axis, "direction", sign, "speed/c", v, "amplitude", amp)
+            assert abs(v - 1) < 0.05, (axis, sign, v)
+            assert 0.95 < amp <= 1.0 + 1e-6, (axis, sign, amp)
+            speeds[axis, sign] = v
+    for sign in (1, -1):
+        assert abs(speeds[1, sign] - speeds[2, sign]) < 1e-9
+        if interp != "weno5c":
+            assert abs(speeds[0, sign] - speeds[1, sign]) < 1e-9
+        assert abs(speeds[0, sign] - speeds[0, -sign]) < 1e-9
+
+
+# ----------------------------------------------------------------------------- totals
+
+def test_totals_match_numpy_sums():
+    """calcTotMoments (update.cu:52-69) and diagn's k (diagn.cu:3-22) of a stepped random state
+    against exactly rounded sums (math.fsum) of the same fields"""
+    shape = (7, 5, 4)
+    o = po.Oracle("D3Q27", shape, "lagr4")
+    o.set_state(*smooth_random_state(shape, seed=12, T0=0.02, u_amp=0.08))
+    o.step(2)
+    F = o.fields()
+    tot = o.totals()
+    u2 = F["ux"] ** 2 + F["uy"] ** 2 + F["uz"] ** 2
+    dV = (2 * math.pi) ** 3 / np.prod(shape)
+    ref = {"mass": math.fsum(F["rho"]), "momentum_x": math.fsum(F["rho"] * F["ux"]),
+           "energy": math.fsum(F["rho"] * (u2 + 3 * F["T"])), "k": math.fsum(0.5 * u2) * dV / (8 * math.pi ** 3)}
+    for key, v in ref.items():
+        assert abs(tot[key] - v) <= 1e-13 * abs(v), (key, tot[key], v)
