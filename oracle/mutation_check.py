@@ -46,6 +46,14 @@ MUTATIONS = {
                            "en += rho * (ux * ux + uy * uy + uz * uz + 2 * T) * 1.0;"),
     # D3Q27 transfer: resting-axis basis A^2 + T1 instead of A^2 - T1 (structs.cuh:536-538)
     "q27_rest_basis_sign": ("const double gx = Ax * Ax - Bx;", "const double gx = Ax * Ax + Bx;"),
+    # mirror boundary: low face reflected to -p instead of -1-p (steps.cu:1008)
+    "mirror_reflect_low": ("if (p < 0) { p = -1 - p; *rev = 1; }", "if (p < 0) { p = -p; *rev = 1; }"),
+    # mirror boundary: high face reflected to 2N-p instead of 2N-1-p (steps.cu:1009)
+    "mirror_reflect_high": ("if (p >= n) { p = n - 1 - p + n; *rev = 1; }", "if (p >= n) { p = n - p + n; *rev = 1; }"),
+    # Cell::mirrX without the velocity sign flip (structs.cuh:553)
+    "mirror_no_u_flip": ("  uT[d] = -uT[d];\n", "  uT[d] = uT[d];\n"),
+    # Cell::mirrX without the population exchange (structs.cuh:555-557)
+    "mirror_no_permutation": ("fn[i] = L->ef[i][d] == 0 ? f[i] : f[L->rev[d][i]];", "fn[i] = f[i];"),
     # calcMoments: temperature without the |u|^2 subtraction of one component (structs.cuh:330)
     "moments_drop_uz2": ("u3 = (u3 - u0 * u0 - u1 * u1 - u2 * u2) * (1. / L->Dim);",
                          "u3 = (u3 - u0 * u0 - u1 * u1) * (1. / L->Dim);"),

@@ -23,6 +23,9 @@
  *   propagate<4> .............. steps.cu:986-1036
  *   LBMpull (D1Q5/D3Q27) ...... steps.cu:356-365, 421-512, 513-564, 582
  *   PonDstepD3Q125 ............ steps.cu:151-250
+ *   mirror boundaries ......... steps.cu:167-190 (PonDstepD3Q125), 1004-1022 (propagate<N>),
+ *                               structs.cuh:552-578 (Cell::mirrX/Y/Z), consts.h:62-64 and
+ *                               lbm.cu:206-244 (reverse tables)
  *   calcTotMoments ............ update.cu:52-69
  *   diagn ..................... diagn.cu:3-22
  *   fill/ind_zip layout ....... structs.cuh:153-166 (x fastest, then y, then z)
@@ -54,6 +57,7 @@ typedef struct {
   double ef[QMAX][3]; /* float velocities ef_host; for D3Q27 the integer table as doubles */
   double w[QMAX];
   double ec1, ec2;
+  int rev[3][QMAX];   /* reverseX/Y/Z: the population whose velocity has axis d negated */
 } olat;
 
 typedef struct {
@@ -61,6 +65,7 @@ typedef struct {
   long nx, ny, nz;
   double dtau, tol_abs, tol_rel;
   int max_iter;
+  int mirror;  /* bit d: mirror boundaries on axis d (the reference's outflow switch) */
 } orc_params;
 
 typedef struct {
@@ -88,7 +93,14 @@ static const int D3Q27_E[27][3] = {
   {0, -1, -1}, {1, -1, -1},
   {-1, -1, -1}};
 
+static void lat_init_tables(olat* L, int lattice);
+static void lat_reverse(olat* L);
 static void lat_init(olat* L, int lattice) {
+  lat_init_tables(L, lattice);
+  lat_reverse(L);
+}
+
+static void lat_init_tables(olat* L, int lattice) {
   memset(L, 0, sizeof(*L));
   L->lattice = lattice;
   if (lattice == ORC_D3Q27) {
@@ -142,6 +154,23 @@ static void lat_init(olat* L, int lattice) {
           }
     }
   }
+}
+
+/* reverseX/Y/Z (consts.h:62-64 for D3Q27, consts.h:131-133 for D1Q5, recomputed for D3Q125 in
+   lbm.cu:206-244): for a velocity with a non-zero component on axis d, the population whose
+   velocity is its mirror image in that axis; the identity otherwise (Cell::mirrX/Y/Z,
+   structs.cuh:552-578, leave those populations in place) */
+static void lat_reverse(olat* L) {
+  for (int d = 0; d < 3; d++)
+    for (int i = 0; i < L->Qn; i++) {
+      L->rev[d][i] = i;
+      if (L->ef[i][d] == 0) continue;
+      for (int j = 0; j < L->Qn; j++) {
+        int same = 1;
+        for (int c = 0; c < 3; c++) same = same && L->ef[j][c] == (c == d ? -L->ef[i][c] : L->ef[i][c]);
+        if (same) { L->rev[d][i] = j; break; }
+      }
+    }
 }
 
 /* ---------------------------------------------------------------- scalar helpers */
@@ -288,6 +317,45 @@ static double node_transfer(const orc_state* S, long x, long y, long z, const do
   return transfer(&S->L, f, uT, base, i);
 }
 
+/* A stencil node under mirror boundaries (propagate<N>, steps.cu:1004-1022): a coordinate
+   below 0 or at or above N on a mirrored axis is reflected about that face (-1-p, or
+   N-1-p+N), then wrapped; the node's cell is then mirrored on each such axis (Cell::mirrX/Y/Z,
+   structs.cuh:552-578: the axis velocity negated, the populations with a non-zero velocity
+   component on the axis exchanged with their mirror images), in the order x, y, z. */
+static long mirror_coord(long p, long n, int* rev) {
+  if (p < 0) { p = -1 - p; *rev = 1; }
+  if (p >= n) { p = n - 1 - p + n; *rev = 1; }
+  return p;
+}
+
+static void mirror_cell(const olat* L, int d, double* f, double* uT) {
+  double fn[QMAX];
+  uT[d] = -uT[d];
+  for (int i = 0; i < L->Qn; i++) fn[i] = L->ef[i][d] == 0 ? f[i] : f[L->rev[d][i]];
+  for (int i = 0; i < L->Qn; i++) f[i] = fn[i];
+}
+
+/* the populations and gauge of node (x, y, z) as the interpolation sees it */
+static void node_mirrored(const orc_state* S, long x, long y, long z, double* f, double* uT) {
+  int rv[3] = {0, 0, 0};
+  long p[3] = {x, y, z};
+  const long n[3] = {S->p.nx, S->p.ny, S->p.nz};
+  for (int d = 0; d < 3; d++)
+    if (S->p.mirror & (1 << d)) p[d] = mirror_coord(p[d], n[d], &rv[d]);
+  const double* c = node(S, p[0], p[1], p[2]);
+  const int Qn = S->L.Qn;
+  for (int k = 0; k < Qn; k++) f[k] = c[k];
+  for (int k = 0; k < 4; k++) uT[k] = c[Qn + 1 + k];
+  for (int d = 0; d < 3; d++)
+    if (rv[d]) mirror_cell(&S->L, d, f, uT);
+}
+
+static double node_transfer_mirrored(const orc_state* S, long x, long y, long z, const double* base, int i) {
+  double f[QMAX], uT[4];
+  node_mirrored(S, x, y, z, f, uT);
+  return transfer(&S->L, f, uT, base, i);
+}
+
 /* LagrEval<3> (steps.cu:924-937) */
 static double lagreval3(const orc_state* S, const double* crd, int lx, int ly, int lz, int i, const double* base) {
   const double sx = crd[0] - lx, sy = crd[1] - ly, sz = crd[2] - lz;
@@ -346,7 +414,9 @@ static double interp_lagr4(const orc_state* S, const double* crd, int i, const d
   for (int ix = 0; ix < 4; ix++)
     for (int iy = 0; iy < 4; iy++)
       for (int iz = 0; iz < 4; iz++)
-        fpr += lagrpol(4, ix, iy, iz, sx, sy, sz) * node_transfer(S, (long)lx + ix, (long)ly + iy, (long)lz + iz, base, i);
+        fpr += lagrpol(4, ix, iy, iz, sx, sy, sz) *
+               (S->p.mirror ? node_transfer_mirrored(S, (long)lx + ix, (long)ly + iy, (long)lz + iz, base, i)
+                            : node_transfer(S, (long)lx + ix, (long)ly + iy, (long)lz + iz, base, i));
   return fpr;
 }
 
@@ -465,10 +535,11 @@ static void cell_step_q125(orc_state* S, long x, long y, long z) {
   ocell c0, cnew;
   load_cell(S, S->ld, idx, &c0);
   zero_reset(L, &c0);
-  const double* fix[27];
+  /* cfix: the 3x3x3 neighbourhood, mirrored at mirror boundaries (steps.cu:163-190) */
+  static __thread double fixf[27][QMAX], fixg[27][4];
   for (int a = 0; a < 3; a++)
     for (int b = 0; b < 3; b++)
-      for (int c = 0; c < 3; c++) fix[9 * a + 3 * b + c] = node(S, x - 1 + a, y - 1 + b, z - 1 + c);
+      for (int c = 0; c < 3; c++) node_mirrored(S, x - 1 + a, y - 1 + b, z - 1 + c, fixf[9 * a + 3 * b + c], fixg[9 * a + 3 * b + c]);
   double vi[QMAX][3];
   int niter = 0;
   while (niter < S->p.max_iter) {
@@ -478,10 +549,7 @@ static void cell_step_q125(orc_state* S, long x, long y, long z) {
     for (int inew = 0; inew < Qn; inew++) {
       double nv[27];
       for (int n = 0; n < 27; n++) {
-        double f[QMAX], g0[4];
-        for (int k = 0; k < Qn; k++) f[k] = fix[n][k];
-        for (int k = 0; k < 4; k++) g0[k] = fix[n][Qn + 1 + k];
-        nv[n] = transfer(L, f, g0, uT, inew);
+        nv[n] = transfer(L, fixf[n], fixg[n], uT, inew);
       }
       const double crd[3] = {(double)x - vi[inew][0], (double)y - vi[inew][1], (double)z - vi[inew][2]};
       const double sh[3] = {crd[0] - (double)(x - 1), crd[1] - (double)(y - 1), crd[2] - (double)(z - 1)};
@@ -513,6 +581,13 @@ void orc_lattice(int lattice, int* e, double* ef, double* w, double* tlat_dim) {
     w[i] = L.w[i];
   }
   tlat_dim[0] = L.TLat; tlat_dim[1] = L.Dim;
+}
+
+/* reverseX/Y/Z tables of a lattice: rev[d*Qn + i] */
+void orc_reverse(int lattice, int* rev) {
+  olat L; lat_init(&L, lattice);
+  for (int d = 0; d < 3; d++)
+    for (int i = 0; i < L.Qn; i++) rev[d * L.Qn + i] = L.rev[d][i];
 }
 
 double orc_transfer(int lattice, const double* f, const double* uT0, const double* base, int inew) {

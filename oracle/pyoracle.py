@@ -33,6 +33,7 @@ class OrcParams(ctypes.Structure):
         ("tol_abs", ctypes.c_double),
         ("tol_rel", ctypes.c_double),
         ("max_iter", ctypes.c_int),
+        ("mirror", ctypes.c_int),
     ]
 
 
@@ -49,6 +50,8 @@ def lib():
         L.orc_qn.argtypes = [ctypes.c_int]
         L.orc_lattice.restype = None
         L.orc_lattice.argtypes = [ctypes.c_int, _I32, _D, _D, _D]
+        L.orc_reverse.restype = None
+        L.orc_reverse.argtypes = [ctypes.c_int, _I32]
         L.orc_transfer.restype = ctypes.c_double
         L.orc_transfer.argtypes = [ctypes.c_int, _D, _D, _D, ctypes.c_int]
         L.orc_lagrpol.restype = ctypes.c_double
@@ -104,6 +107,13 @@ def lattice(name):
     return e, ef, w, float(td[0]), float(td[1])
 
 
+def reverse(name):
+    """reverseX/Y/Z of the lattice as int[3][Qn]"""
+    r = np.zeros((3, QN[name]), np.int32)
+    lib().orc_reverse(LATTICES[name], r.ctypes.data_as(_I32))
+    return r
+
+
 def transfer(name, f, uT0, base, inew):
     f, uT0, base = _c(f), _c(uT0), _c(base)
     return lib().orc_transfer(LATTICES[name], _p(f), _p(uT0), _p(base), int(inew))
@@ -153,9 +163,12 @@ def mlp_grad(params, X, T, bf16=True):
 class Oracle:
     """CPU restatement of the reference step on one grid."""
 
-    def __init__(self, lattice, shape, interp="weno5", dtau=1.0 / 0.9, tol_abs=1e-12, tol_rel=1e-10, max_iter=100):
+    def __init__(self, lattice, shape, interp="weno5", dtau=1.0 / 0.9, tol_abs=1e-12, tol_rel=1e-10, max_iter=100,
+                 mirror=0):
+        """mirror: bit d set = mirror boundaries on axis d (Lagrange-4 and D3Q125 only, as in the
+        reference, whose WENO5 interpolation has no mirror path)"""
         p = OrcParams(LATTICES[lattice], INTERPS[interp], int(shape[0]), int(shape[1]), int(shape[2]), float(dtau),
-                      float(tol_abs), float(tol_rel), int(max_iter))
+                      float(tol_abs), float(tol_rel), int(max_iter), int(mirror))
         self._h = lib().orc_create(ctypes.byref(p))
         if not self._h:
             raise MemoryError("orc_create failed")

@@ -522,3 +522,94 @@ def test_totals_match_numpy_sums():
            "energy": math.fsum(F["rho"] * (u2 + 3 * F["T"])), "k": math.fsum(0.5 * u2) * dV / (8 * math.pi ** 3)}
     for key, v in ref.items():
         assert abs(tot[key] - v) <= 1e-13 * abs(v), (key, tot[key], v)
+
+
+# ----------------------------------------------------------------------------- mirror boundaries
+
+# reverseX/Y/Z tables as the reference writes them (consts.h:62-64 for D3Q27, 131-133 for D1Q5)
+REF_REVERSE_D3Q27 = [
+    [0, 8, 2, 3, 9, 10, 6, 11, 1, 4, 5, 7, 12, 13, 20, 21, 16, 22, 18, 23, 14, 15, 17, 19, 24, 26, 25],
+    [0, 1, 12, 3, 14, 5, 13, 15, 8, 20, 10, 21, 2, 6, 4, 7, 16, 17, 24, 25, 9, 11, 22, 26, 18, 19, 23],
+    [0, 1, 2, 16, 4, 17, 18, 19, 8, 9, 22, 23, 12, 24, 14, 25, 3, 5, 6, 7, 20, 26, 10, 11, 13, 15, 21]]
+REF_REVERSE_D1Q5 = [[0, 2, 1, 4, 3], [0, 1, 2, 3, 4], [0, 1, 2, 3, 4]]
+
+
+def test_reverse_tables():
+    """the oracle's mirror permutations: the reference's literal tables for D3Q27 and D1Q5; for
+    D3Q125 (recomputed at start-up in lbm.cu:206-244) the defining property: an involution that
+    maps a velocity with a non-zero axis-d component to its mirror image and fixes the others"""
+    assert po.reverse("D3Q27").tolist() == REF_REVERSE_D3Q27
+    assert po.reverse("D1Q5").tolist() == REF_REVERSE_D1Q5
+    _, ef, _, _, _ = po.lattice("D3Q125")
+    r = po.reverse("D3Q125")
+    for d in range(3):
+        assert (r[d][r[d]] == np.arange(125)).all()
+        m = ef.copy()
+        m[:, d] *= -1
+        moved = ef[:, d] != 0
+        assert np.array_equal(ef[r[d]][moved], m[moved])
+        assert (r[d][~moved] == np.arange(125)[~moved]).all()
+
+
+def _mirror_image(lattice, f, fields, shape, axis):
+    """state and populations on the domain doubled along `axis` (mirror image in the second half)"""
+    nx, ny, nz = shape
+    rev = po.reverse(lattice)[axis]
+    q = f.shape[0]
+    f4 = f.reshape(q, nz, ny, nx)
+    ax = {0: 3, 1: 2, 2: 1}[axis]
+    f_img = np.flip(f4[rev], axis=ax)
+    f2 = np.concatenate([f4, f_img], axis=ax).reshape(q, -1)
+    out = []
+    for k, a in zip(("rho", "ux", "uy", "uz", "T"), fields):
+        a3 = a.reshape(nz, ny, nx)
+        img = np.flip(a3, axis=ax - 1) * (-1.0 if k == "u" + "xyz"[axis] else 1.0)
+        out.append(np.concatenate([a3, img], axis=ax - 1).reshape(-1))
+    return f2, out
+
+
+@pytest.mark.parametrize("lattice,shape,axes,interp", [("D1Q5", (40, 1, 1), (0,), "lagr4"),
+                                                       ("D3Q27", (8, 6, 5), (0, 1, 2), "lagr4"),
+                                                       ("D3Q125", (5, 4, 3), (0, 2), "weno5")])
+def test_mirror_walls_equal_the_doubled_periodic_domain(lattice, shape, axes, interp):
+    """Mirror boundaries (steps.cu:167-190, 1004-1022; structs.cuh:552-578) on a grid equal a
+    periodic grid doubled along each mirrored axis that holds the mirror image of the state
+    (positions reflected about the face at -1/2, the axis velocity negated, the populations
+    permuted by reverseX/Y/Z).  One step from a mirror-symmetric doubled state reads exactly the
+    same node data, so it agrees bit for bit; later steps agree to round-off (the doubled grid's
+    image cells round their own arithmetic).  A wrong reflection (-p instead of -1-p), a velocity
+    left unnegated or populations left in place each differ by orders of magnitude more."""
+    rng = np.random.default_rng(37)
+    fields = smooth_random_state(shape, seed=38, T0=0.02, dims=1 if lattice == "D1Q5" else 3)
+    o = po.Oracle(lattice, shape, interp)
+    o.set_state(*fields)
+    f = o.populations() * (1 + 1e-3 * rng.uniform(-1, 1, (po.QN[lattice], int(np.prod(shape)))))
+    f2, big = f, list(fields)
+    bshape = list(shape)
+    for ax in axes:
+        f2, big = _mirror_image(lattice, f2, big, tuple(bshape), ax)
+        bshape[ax] *= 2
+    mirror = sum(1 << ax for ax in axes)
+    for steps, tol in ((1, 0.0), (4, 1e-11)):
+        a = po.Oracle(lattice, shape, interp, mirror=mirror)
+        a.set_populations(f, *fields)
+        a.step(steps)
+        b = po.Oracle(lattice, tuple(bshape), interp)
+        b.set_populations(f2, *big)
+        b.step(steps)
+        ra, rb = a.fields(), b.fields()
+        nx, ny, nz = shape
+        bx, by, bz = bshape
+        for k in ("rho", "ux", "uy", "uz", "T"):
+            sub = rb[k].reshape(bz, by, bx)[:nz, :ny, :nx].reshape(-1)
+            scale = max(np.abs(sub).max(), 1e-300)
+            err = np.abs(ra[k] - sub).max() / scale
+            assert err <= tol, (steps, k, err)
+        if steps == 1:
+            sub = rb["niter"].reshape(bz, by, bx)[:nz, :ny, :nx].reshape(-1)
+            assert np.array_equal(ra["niter"], sub)
+        # the mirrored walls do change the result (the pin is not vacuous)
+        p = po.Oracle(lattice, shape, interp)
+        p.set_populations(f, *fields)
+        p.step(steps)
+        assert np.abs(p.fields()["rho"] - ra["rho"]).max() > 1e-6
