@@ -202,6 +202,14 @@ pond_status pond_mlp_set_params(pond_ctx* ctx, const float* params, const float*
    scales applied), through the fused MFMA inference kernel */
 pond_status pond_mlp_forward(pond_ctx* ctx, const float* X, int64_t B, float* Y);
 
+/* pond_mlp_forward that allocates nothing, for callers that cannot allocate device memory
+   (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  No
+   counterpart in the reference.  The device staging of X and Y is a static array of the library,
+   one per GPU, held until the process ends: B is at most 16384 rows (larger B returns
+   POND_ERR_INVALID).  Every call on one GPU uses the same array, so calls that overlap on one
+   GPU (from several threads or contexts) race.  Call with the context's device current. */
+pond_status pond_mlp_forward_noalloc(pond_ctx* ctx, const float* X, int64_t B, float* Y);
+
 /* the network's predicted start for every cell of the current state, as the step uses it:
    out[4][n] (x fastest) = (du_x/sqrt(T), du_y/sqrt(T), du_z/sqrt(T), dT/T) relative to the
    cell's gauge, computed by the fused inference kernel from the 3x3x3 neighbourhoods.  No

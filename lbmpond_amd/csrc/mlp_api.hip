@@ -3,6 +3,7 @@
 // parameter I/O, and the forward/gradient entry points used to check the kernels.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -134,6 +135,41 @@ pond_status mlp_fwd_bwd(pond_ctx* ctx, int B, bool bias_grad) {
 }
 
 MlpState state_view(pond_ctx* ctx, int b) { return MlpState{ctx->g, ctx->sp[b].m, ctx->sp[b].T}; }
+
+// ---- device scratch of pond_mlp_forward_noalloc: X rows, then the output planes, then the
+// output rows, for at most kNoAllocRows rows.  One copy per GPU, held until the process ends.
+constexpr int64_t kNoAllocRows = 16384;
+constexpr size_t kNoAllocFloats = (size_t)kNoAllocRows * (kMlpIn + 2 * kMlpOut);
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+constexpr int kMaxDevices = 64;
+float* g_scratch_dev[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// the device address of this GPU's scratch array (looked up once per device)
+hipError_t scratch_ptr(float** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[dev] = static_cast<float*>(ptr);
+  }
+  *out = g_scratch_dev[dev];
+  return hipSuccess;
+}
+
+// output planes [o][b] -> rows [b][o]
+__global__ void k_planes_to_rows(const float* __restrict__ planes, int64_t n, float* __restrict__ rows) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * kMlpOut) return;
+  const int64_t b = i / kMlpOut;
+  const int o = (int)(i % kMlpOut);
+  rows[i] = planes[(int64_t)o * n + b];
+}
 
 }  // namespace
 
@@ -417,6 +453,33 @@ pond_status pond_mlp_grad(pond_ctx* ctx, const float* X, const float* T, int64_t
     MLP_HIP(ctx, hipMemcpy(&l, m.loss, sizeof(float), hipMemcpyDeviceToHost));
     *loss = l / (float)B;
   }
+  return POND_OK;
+}
+
+// pond_mlp_forward without any allocation: the device staging of X and Y lives in a static
+// device array (kNoAllocRows rows at most), shared by every call on the same GPU, so calls that
+// overlap on one GPU race.  Run with the context's device current.
+pond_status pond_mlp_forward_noalloc(pond_ctx* ctx, const float* X, int64_t B, float* Y) {
+  if (!ctx || !X || !Y || B < 1) return POND_ERR_INVALID;
+  if (B > kNoAllocRows) {
+    ctx->err = "pond_mlp_forward_noalloc: B = " + std::to_string(B) + " exceeds the maximum of " +
+               std::to_string(kNoAllocRows) + " rows";
+    return POND_ERR_INVALID;
+  }
+  MlpCtx& m = ctx->mlp;
+  if (!m.params) { ctx->err = "pond_mlp_forward_noalloc: no network"; return POND_ERR_STATE; }
+  float* base = nullptr;
+  MLP_HIP(ctx, scratch_ptr(&base));
+  float* dX = base;
+  float* dP = base + (size_t)kNoAllocRows * kMlpIn;
+  float* dR = dP + (size_t)kNoAllocRows * kMlpOut;
+  MLP_HIP(ctx, hipMemcpyAsync(dX, X, (size_t)B * kMlpIn * 4, hipMemcpyHostToDevice, ctx->stream));
+  MLP_HIP(ctx, mlp_launch_infer_matrix(pond_mlp_weights(ctx), dX, B, dP, ctx->stream));
+  const int64_t nv = B * kMlpOut;
+  hipLaunchKernelGGL(k_planes_to_rows, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, ctx->stream, dP, B, dR);
+  MLP_HIP(ctx, hipGetLastError());
+  MLP_HIP(ctx, hipMemcpyAsync(Y, dR, (size_t)nv * 4, hipMemcpyDeviceToHost, ctx->stream));
+  MLP_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return POND_OK;
 }
 

@@ -119,6 +119,7 @@ SIGNATURES = {
     "pond_mlp_get_params": (ctypes.c_int, [_CTX, _F, _F]),
     "pond_mlp_set_params": (ctypes.c_int, [_CTX, _F, _F]),
     "pond_mlp_forward": (ctypes.c_int, [_CTX, _F, ctypes.c_int64, _F]),
+    "pond_mlp_forward_noalloc": (ctypes.c_int, [_CTX, _F, ctypes.c_int64, _F]),
     "pond_mlp_predict": (ctypes.c_int, [_CTX, _F]),
     "pond_mlp_grad": (ctypes.c_int, [_CTX, _F, _F, ctypes.c_int64, _F, _F]),
 }
@@ -349,6 +350,14 @@ class Pond:
         Y = np.zeros((B, 4), np.float32)
         self._check(lib().pond_mlp_forward(self._h, X.ctypes.data_as(_F), B, Y.ctypes.data_as(_F)),
                     "pond_mlp_forward")
+        return Y
+
+    def mlp_forward_noalloc(self, X):
+        """mlp_forward through the entry point that allocates no device memory (at most 16384 rows)"""
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((X.shape[0], 4), np.float32)
+        self._check(lib().pond_mlp_forward_noalloc(self._h, X.ctypes.data_as(_F), X.shape[0], Y.ctypes.data_as(_F)),
+                    "pond_mlp_forward_noalloc")
         return Y
 
     def mlp_predict(self):
