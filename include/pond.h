@@ -96,6 +96,14 @@ typedef struct {
   /* POND_SPLIT with use_mlp: largest predicted change (|du|/sqrt(T), |dT|/T) taken as a
      start; larger predictions start from the previous gauge.  Default 0.3. */
   double learned_trust;
+  /* Mirror boundaries, bit d for axis d (1: x, 2: y, 4: z); 0 (default): periodic.  The
+     reference's compile-time outflow switch (steps.cu:167-190, 1004-1022): a stencil node
+     beyond a mirrored face is the mirror image of the cell inside (position reflected about the
+     face, axis velocity negated, populations exchanged with their mirror images, Cell::mirrX/Y/Z,
+     structs.cuh:552-578).  The reference has it in propagate<N> and PonDstepD3Q125 only, so it
+     applies to POND_LAGR4 (D1Q5, D3Q27) and to D3Q125; WENO5 interpolation with a mirror is
+     rejected (POND_ERR_INVALID). */
+  int mirror;
 } pond_params;
 
 typedef struct {
@@ -106,6 +114,9 @@ typedef struct {
   int32_t niter_max; /* max gauge iterations, last step (steps.cu:513) */
   double niter_mean; /* mean gauge iterations per cell, last step */
   int64_t step;      /* steps taken (LBMParams::iStep) */
+  double max_departure; /* largest population displacement per step along an axis, in cells:
+                           max over cells and axes of |u_d| + sqrt(T/TLat)*max|e_d| (the
+                           interpolation stencils reach this far plus their own width) */
 } pond_diag;
 
 typedef struct {
@@ -142,6 +153,13 @@ pond_status pond_set_state(pond_ctx* ctx, const double* rho, const double* ux, c
 /* arbitrary full state: f plane-major (Qn*N) plus the five moment planes */
 pond_status pond_set_populations(pond_ctx* ctx, const double* f, const double* rho, const double* ux,
                                  const double* uy, const double* uz, const double* T);
+
+/* the current state of src (populations, moments, per-cell iteration counts, step count) into
+   dst, on the device, converting between FP64 and split storage (FP64 -> FP32 rounds to
+   nearest).  Same lattice and grid; the interpolator and parameters of dst are its own.  No
+   counterpart in the reference (one global state); it lets a run switch precision, and lets one
+   step of two paths start from the same state. */
+pond_status pond_copy_state(pond_ctx* dst, pond_ctx* src);
 
 /* nsteps PonD steps (calcStep without I/O).  Asynchronous on the ctx stream; returns after
    the launches are queued.  pond_sync / any get call waits. */

@@ -53,9 +53,44 @@ __global__ void k_set_split(const double* __restrict__ m, float* __restrict__ f,
   T[c] = m[(int64_t)4 * plane + c];
 }
 
+// one state in another context's layout (pond_copy_state): populations and moments of every
+// cell from an FP64 or split source into an FP64 or split destination (FP64 -> FP32 rounds to
+// nearest, FP32 -> FP64 is exact)
+struct StateView {
+  const double* f64;  // FP64 populations (f64_fidx layout) and moment planes ...
+  const double* m64;
+  const float* f32;   // ... or split populations (split_fidx layout), (rho, u) planes and T
+  const float* m32;
+  const double* T;
+};
+struct StateOut {
+  double* f64;
+  double* m64;
+  float* f32;
+  float* m32;
+  double* T;
+};
+
+__global__ void k_copy_state(StateView in, StateOut out, int64_t n, int64_t plane, int lattice, int qn) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  for (int i = 0; i < qn; i++) {
+    const double v = in.f64 ? in.f64[f64_fidx(lattice, i, c, plane)] : (double)in.f32[split_fidx(lattice, i, c, plane)];
+    if (out.f64) out.f64[f64_fidx(lattice, i, c, plane)] = v;
+    else out.f32[split_fidx(lattice, i, c, plane)] = (float)v;
+  }
+  for (int k = 0; k < 5; k++) {
+    const double v = in.m64 ? in.m64[(int64_t)k * plane + c] : k < 4 ? (double)in.m32[(int64_t)k * plane + c] : in.T[c];
+    if (out.m64) out.m64[(int64_t)k * plane + c] = v;
+    else if (k < 4) out.m32[(int64_t)k * plane + c] = (float)v;
+    else out.T[c] = v;
+  }
+}
+
 // block-level deterministic partial sums: mass, momentum_x, energy, k, niter sum; niter max
 struct RedOut {
   double s[5];
+  double vmax;  // largest |u_d| + sqrt(|T|/TLat)*emax
   int imax;
 };
 
@@ -83,13 +118,31 @@ __device__ int block_max(int v, int* sh) {
   return t;
 }
 
+__device__ double block_maxd(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) t = fmax(t, sh[i]);
+  return t;
+}
+
+// the displacement bound of one cell: max_d |u_d| + sqrt(|T|/TLat) * emax
+__device__ __forceinline__ double departure(double ux, double uy, double uz, double T, double inv_tlat, double emax) {
+  return fmax(fmax(fabs(ux), fabs(uy)), fabs(uz)) + sqrt(fabs(T) * inv_tlat) * emax;
+}
+
 // calcTotMoments (update.cu:52-69) + diagn (diagn.cu:3-22) + NiterMax, for the FP64 state
 __global__ __launch_bounds__(256) void k_reduce_f64(const double* __restrict__ m, const int* __restrict__ niter,
                                                     int64_t n, int64_t plane, double dim, double cx, double cy,
-                                                    double cz, RedOut* __restrict__ out) {
+                                                    double cz, double inv_tlat, double emax, RedOut* __restrict__ out) {
   __shared__ double shd[4];
   __shared__ int shi[4];
   double s[5] = {0, 0, 0, 0, 0};
+  double vmax = 0;
   int imax = 0;
   const double d8pi3 = 1.0 / (8 * M_PI * M_PI * M_PI);
   for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
@@ -100,21 +153,24 @@ __global__ __launch_bounds__(256) void k_reduce_f64(const double* __restrict__ m
     s[3] += 0.5 * d8pi3 * (ux * ux + uy * uy + uz * uz) * cx * cy * cz;
     s[4] += (double)niter[c];
     imax = max(imax, niter[c]);
+    vmax = fmax(vmax, departure(ux, uy, uz, T, inv_tlat, emax));
   }
   RedOut r;
   for (int q = 0; q < 5; q++) r.s[q] = block_sum(s[q], shd);
   r.imax = block_max(imax, shi);
+  r.vmax = block_maxd(vmax, shd);
   if (threadIdx.x == 0) out[blockIdx.x] = r;
 }
 
 // the same reductions for the split layout (FP32 rho,u and FP64 T)
 __global__ __launch_bounds__(256) void k_reduce_split(const float* __restrict__ m, const double* __restrict__ Tp,
                                                       const int* __restrict__ niter, int64_t n, int64_t plane,
-                                                      double dim, double cx, double cy, double cz,
-                                                      RedOut* __restrict__ out) {
+                                                      double dim, double cx, double cy, double cz, double inv_tlat,
+                                                      double emax, RedOut* __restrict__ out) {
   __shared__ double shd[4];
   __shared__ int shi[4];
   double s[5] = {0, 0, 0, 0, 0};
+  double vmax = 0;
   int imax = 0;
   const double d8pi3 = 1.0 / (8 * M_PI * M_PI * M_PI);
   for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
@@ -125,10 +181,12 @@ __global__ __launch_bounds__(256) void k_reduce_split(const float* __restrict__ 
     s[3] += 0.5 * d8pi3 * (ux * ux + uy * uy + uz * uz) * cx * cy * cz;
     s[4] += (double)niter[c];
     imax = max(imax, niter[c]);
+    vmax = fmax(vmax, departure(ux, uy, uz, T, inv_tlat, emax));
   }
   RedOut r;
   for (int q = 0; q < 5; q++) r.s[q] = block_sum(s[q], shd);
   r.imax = block_max(imax, shi);
+  r.vmax = block_maxd(vmax, shd);
   if (threadIdx.x == 0) out[blockIdx.x] = r;
 }
 
@@ -138,6 +196,7 @@ __global__ void k_reduce_final(const RedOut* __restrict__ part, int nb, RedOut* 
   for (int b = 0; b < nb; b++) {
     for (int q = 0; q < 5; q++) r.s[q] += part[b].s[q];
     r.imax = max(r.imax, part[b].imax);
+    r.vmax = fmax(r.vmax, part[b].vmax);
   }
   *out = r;
 }
@@ -266,10 +325,15 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
   if (p->pass1_iters < -1 || p->pass1_iters == 0)
     return bail(POND_ERR_INVALID, "pond_create: pass1_iters must be -1 or >= 1");
   if (!(p->learned_trust >= 0)) return bail(POND_ERR_INVALID, "pond_create: learned_trust must be >= 0");
+  if (p->mirror < 0 || p->mirror > 7) return bail(POND_ERR_INVALID, "pond_create: mirror must be a mask of axes 0..7");
+  if (p->mirror && p->lattice != POND_D3Q125 && p->interp != POND_LAGR4)
+    return bail(POND_ERR_INVALID,
+                "pond_create: mirror boundaries apply to POND_LAGR4 and D3Q125 (the reference's WENO5 has no mirror path)");
   // row patches where the grid divides into them (ignored elsewhere)
   const int rp = p->tile_patch;
   g.rp = (rp > 0 && g.nx % 64 == 0 && g.ny % rp == 0 && g.nz % rp == 0) ? rp : 0;
   g.ntiles = g.rp > 0 ? (int64_t)(g.nx / 64) * g.ny * g.nz : (g.n + 63) / 64;
+  g.mirror = p->mirror;
   // the split step addresses nine FP32 population planes through one 32-bit buffer offset
   // the split step addresses two population quad planes (16 bytes per cell each) through one
   // 32-bit buffer offset
@@ -510,6 +574,37 @@ pond_status pond_set_populations(pond_ctx* ctx, const double* f, const double* r
   return POND_OK;
 }
 
+pond_status pond_copy_state(pond_ctx* dst, pond_ctx* src) {
+  if (!dst || !src) return POND_ERR_INVALID;
+  if (!src->has_state) return fail(dst, POND_ERR_STATE, "pond_copy_state: the source has no state");
+  if (dst->p.lattice != src->p.lattice || dst->g.nx != src->g.nx || dst->g.ny != src->g.ny || dst->g.nz != src->g.nz)
+    return fail(dst, POND_ERR_INVALID, "pond_copy_state: lattice and grid must match");
+  if (dst == src) return POND_OK;
+  const Grid& g = dst->g;
+  POND_HIP(src, hipStreamSynchronize(src->stream));
+  POND_HIP(dst, hipStreamSynchronize(dst->stream));
+  StateView in{};
+  if (src->split) { in.f32 = src->sp[src->cur].f; in.m32 = src->sp[src->cur].m; in.T = src->sp[src->cur].T; }
+  else { in.f64 = src->f64[src->cur].f; in.m64 = src->f64[src->cur].m; }
+  const unsigned blocks = (unsigned)((g.n + 255) / 256);
+  // both buffers of the destination hold the state, as after pond_set_populations
+  for (int b = 0; b < 2; b++) {
+    StateOut out{};
+    if (dst->split) { out.f32 = dst->sp[b].f; out.m32 = dst->sp[b].m; out.T = dst->sp[b].T; }
+    else { out.f64 = dst->f64[b].f; out.m64 = dst->f64[b].m; }
+    hipLaunchKernelGGL(k_copy_state, dim3(blocks), dim3(256), 0, dst->stream, in, out, g.n, g.plane, dst->p.lattice,
+                       dst->Qn);
+    POND_HIP(dst, hipGetLastError());
+  }
+  POND_HIP(dst, hipMemcpyAsync(dst->niter, src->niter, (size_t)g.plane * sizeof(int), hipMemcpyDeviceToDevice,
+                               dst->stream));
+  POND_HIP(dst, hipStreamSynchronize(dst->stream));
+  dst->cur = 0;
+  dst->has_state = true;
+  dst->step = src->step;
+  return POND_OK;
+}
+
 // the pending-cell list as the step kernels see it (pond_ctx::pend layout)
 static SplitPending pending_view(pond_ctx* ctx) {
   const int64_t cap = ctx->pend_cap;
@@ -640,12 +735,18 @@ pond_status pond_diagnostics(pond_ctx* ctx, pond_diag* out) {
   const Grid& g = ctx->g;
   auto* part = reinterpret_cast<RedOut*>(ctx->red);
   const int nb = (int)std::min<int64_t>(kRedBlocks, (g.n + 255) / 256);
+  // largest velocity component of the lattice, in thermal units
+  double emax = 0;
+  for (int i = 0; i < ctx->Qn; i++)
+    for (int d = 0; d < 3; d++) emax = std::max(emax, std::fabs(ctx->L.ef[i][d]));
   if (ctx->split)
     hipLaunchKernelGGL(k_reduce_split, dim3(nb), dim3(256), 0, ctx->stream, ctx->sp[ctx->cur].m, ctx->sp[ctx->cur].T,
-                       ctx->niter, g.n, g.plane, ctx->L.Dim, 2 * M_PI / g.nx, 2 * M_PI / g.ny, 2 * M_PI / g.nz, part);
+                       ctx->niter, g.n, g.plane, ctx->L.Dim, 2 * M_PI / g.nx, 2 * M_PI / g.ny, 2 * M_PI / g.nz,
+                       1.0 / ctx->L.TLat, emax, part);
   else
     hipLaunchKernelGGL(k_reduce_f64, dim3(nb), dim3(256), 0, ctx->stream, ctx->f64[ctx->cur].m, ctx->niter, g.n,
-                       g.plane, ctx->L.Dim, 2 * M_PI / g.nx, 2 * M_PI / g.ny, 2 * M_PI / g.nz, part);
+                       g.plane, ctx->L.Dim, 2 * M_PI / g.nx, 2 * M_PI / g.ny, 2 * M_PI / g.nz, 1.0 / ctx->L.TLat, emax,
+                       part);
   POND_HIP(ctx, hipGetLastError());
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, ctx->stream, part, nb, part + kRedBlocks);
   POND_HIP(ctx, hipGetLastError());
@@ -660,6 +761,7 @@ pond_status pond_diagnostics(pond_ctx* ctx, pond_diag* out) {
   out->niter_mean = ctx->step > 0 ? r.s[4] / (double)g.n : 0.0;
   out->niter_max = ctx->step > 0 ? r.imax : 0;
   out->step = ctx->step;
+  out->max_departure = r.vmax;
   return POND_OK;
 }
 

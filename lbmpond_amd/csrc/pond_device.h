@@ -19,6 +19,8 @@ struct Grid {
   // instead of y fastest over the whole plane; 0 = plain row order.  Needs nx % 64 == 0,
   // ny % rp == 0 and nz % rp == 0.
   int rp;
+  // mirror-boundary axes, bit d for axis d (0: periodic in every axis)
+  int mirror;
 };
 
 // Parameters of one step, passed by value to the kernels.
@@ -77,6 +79,21 @@ __device__ __forceinline__ int wrapi(int v, int n) {
   int r = v % n;
   return r < 0 ? r + n : r;
 }
+
+// Mirror boundaries (propagate<N>, steps.cu:1004-1017): on a mirrored axis a coordinate below 0
+// or at or above n is reflected about that face (-1-v, or n-1-v+n) and `rev` is set; then the
+// periodic wrap, as in the reference (which wraps after reflecting).
+__device__ __forceinline__ int mirror_wrap(int v, int n, bool mirrored, bool& rev) {
+  rev = false;
+  if (mirrored) {
+    if (v < 0) { v = -1 - v; rev = true; }
+    if (v >= n) { v = n - 1 - v + n; rev = true; }
+  }
+  return wrapi(v, n);
+}
+
+// mirror image of a D1Q5 / D3Q125 per-axis velocity class {0, +e1, -e1, +e2, -e2}
+constexpr int kMir5[5] = {0, 2, 1, 4, 3};
 
 // floor(v) as a stencil coordinate.  A double->int conversion of NaN or of a value outside the
 // int range is undefined (LLVM: poison), and a poisoned coordinate turns into an arbitrary

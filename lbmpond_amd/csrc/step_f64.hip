@@ -177,12 +177,8 @@ __device__ __forceinline__ void node_f27(const NodeBuf& s, uint32_t boff, double
 
 // basis values of the node at byte offset boff for target t (structs.cuh:530-538).
 // e_iold*sqrt(|T0|)*sign(T0) for e_iold = -1, 0, +1 is exactly {-P, 0*P, P}.
-template <class SRC>
-__device__ __forceinline__ bool basis27(const SRC& s, uint32_t boff, const Tgt27& t, double sqTb, double gx[3],
-                                        double gy[3], double gz[3]) {
-  double u0x, u0y, u0z, Pn;
-  node_g4(s, boff, u0x, u0y, u0z, Pn);
-  const bool nok = node_ok(s, boff);
+__device__ __forceinline__ bool basis27u(double u0x, double u0y, double u0z, double Pn, bool nok, const Tgt27& t,
+                                         double sqTb, double gx[3], double gy[3], double gz[3]) {
   const double Dx = (t.u1x - u0x) * sqTb;
   const double Dy = (t.u1y - u0y) * sqTb;
   const double Dz = (t.u1z - u0z) * sqTb;
@@ -197,6 +193,14 @@ __device__ __forceinline__ bool basis27(const SRC& s, uint32_t boff, const Tgt27
     safe = safe && fabs(gx[j]) < kSafe && fabs(gy[j]) < kSafe && fabs(gz[j]) < kSafe;
   }
   return safe;
+}
+
+template <class SRC>
+__device__ __forceinline__ bool basis27(const SRC& s, uint32_t boff, const Tgt27& t, double sqTb, double gx[3],
+                                        double gy[3], double gz[3]) {
+  double u0x, u0y, u0z, Pn;
+  node_g4(s, boff, u0x, u0y, u0z, Pn);
+  return basis27u(u0x, u0y, u0z, Pn, node_ok(s, boff), t, sqTb, gx, gy, gz);
 }
 
 // gauge transfer of the node at byte offset boff to target t (structs.cuh:525-548)
@@ -231,6 +235,48 @@ __device__ __forceinline__ double transfer27(const SRC& s, uint32_t boff, const 
   return fnew;
 }
 
+// transfer27 of the node's mirror image on the axes set in mx, my, mz (Cell::mirrX/Y/Z,
+// structs.cuh:552-578): the axis velocity negated and every population exchanged with its mirror
+// image (kIdx27 with that component negated), then the same term loop
+template <class SRC>
+__device__ double transfer27_mirrored(const SRC& s, uint32_t boff, const Tgt27& t, double sqTb, bool mx, bool my,
+                                      bool mz) {
+  double u0x, u0y, u0z, Pn;
+  node_g4(s, boff, u0x, u0y, u0z, Pn);
+  if (mx) u0x = -u0x;
+  if (my) u0y = -u0y;
+  if (mz) u0z = -u0z;
+  double gx[3], gy[3], gz[3];
+  const bool safe = basis27u(u0x, u0y, u0z, Pn, node_ok(s, boff), t, sqTb, gx, gy, gz);
+  double p2[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) p2[a][b] = t.W * gx[a] * gy[b];
+  double f0[27], f[27];
+  node_f27(s, boff, f0);
+#pragma unroll
+  for (int i = 0; i < 27; i++) {
+    const int jx = kE27[i][0] + 1, jy = kE27[i][1] + 1, jz = kE27[i][2] + 1;
+    // f[i] = f0 of the mirrored velocity, as selects over compile-time indices
+    const double a0 = mx ? f0[kIdx27.v[2 - jx][jy][jz]] : f0[i];
+    const double a1 = mx ? f0[kIdx27.v[2 - jx][2 - jy][jz]] : f0[kIdx27.v[jx][2 - jy][jz]];
+    const double b0 = my ? a1 : a0;
+    const double a2 = mx ? f0[kIdx27.v[2 - jx][jy][2 - jz]] : f0[kIdx27.v[jx][jy][2 - jz]];
+    const double a3 = mx ? f0[kIdx27.v[2 - jx][2 - jy][2 - jz]] : f0[kIdx27.v[jx][2 - jy][2 - jz]];
+    const double b1 = my ? a3 : a2;
+    f[i] = mz ? b1 : b0;
+  }
+  double fnew = 0;
+#pragma unroll
+  for (int iold = 0; iold < 27; iold++) {
+    const int jx = kE27[iold][0] + 1, jy = kE27[iold][1] + 1, jz = kE27[iold][2] + 1;
+    const double fadd = p2[jx][jy] * gz[jz] * f[iold];
+    if (safe || finite_d(fadd)) fnew += fadd;
+  }
+  return fnew;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Gauge transfer, D1Q5 (structs.cuh:352-395)
 
@@ -261,9 +307,20 @@ __device__ __forceinline__ Tgt5 make_tgt5(int inew, double u1x, double T1) {
   return t;
 }
 
-template <class SRC>
-__device__ __forceinline__ double transfer5(const SRC& s, uint32_t boff, const Tgt5& t) {
-  const double u0x = node_m(s, 1, boff);
+// MIRX: the node's mirror image in x (Cell::mirrX, structs.cuh:552-559): u0x negated and the
+// populations +e/-e exchanged
+template <bool MIRX = false, class SRC>
+__device__ __forceinline__ double transfer5(const SRC& s, uint32_t boff, const Tgt5& t, bool mx = false) {
+  double u0x = node_m(s, 1, boff);
+  double fm[5];
+  if (MIRX) {
+    if (mx) u0x = -u0x;
+    double f0[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) f0[i] = node_f(s, i, boff);
+#pragma unroll
+    for (int i = 0; i < 5; i++) fm[i] = mx ? f0[kMir5[i]] : f0[i];
+  }
   const double s0 = node_P(s, boff);  // sqrts(T0/1.0) = sqrt(|T0|)*sign(T0)
   const double et = t.et;
   const double difu = t.u1x - u0x;
@@ -287,7 +344,7 @@ __device__ __forceinline__ double transfer5(const SRC& s, uint32_t boff, const T
     dm4 = dm4 * dm4 * dm4 * dm4;
     dm4 *= 0;
     const double gx = K4 * difu * difu * difu * difu + K3 * difu * difu * difu + K2 * difu * difu + K1 * difu + K0 - dm4;
-    const double fadd = t.wk * gx * node_f(s, iold, boff);
+    const double fadd = t.wk * gx * (MIRX ? fm[iold] : node_f(s, iold, boff));
     if (finite_d(fadd)) fnew += fadd;
   }
   return fnew;
@@ -309,6 +366,10 @@ struct Lat<kD3Q27> {
   __device__ static double xfer(const SRC& s, uint32_t boff, const Tgt& t, double sqTb) {
     return transfer27(s, boff, t, sqTb);
   }
+  template <class SRC>
+  __device__ static double xferm(const SRC& s, uint32_t boff, const Tgt& t, double sqTb, bool mx, bool my, bool mz) {
+    return transfer27_mirrored(s, boff, t, sqTb, mx, my, mz);
+  }
 };
 
 template <>
@@ -320,6 +381,11 @@ struct Lat<kD1Q5> {
   template <class SRC>
   __device__ static double xfer(const SRC& s, uint32_t boff, const Tgt& t, double) {
     return transfer5(s, boff, t);
+  }
+  // D1Q5 velocities have no y or z component: a mirror in y or z only negates the unused uy, uz
+  template <class SRC>
+  __device__ static double xferm(const SRC& s, uint32_t boff, const Tgt& t, double, bool mx, bool, bool) {
+    return transfer5<true>(s, boff, t, mx);
   }
 };
 
@@ -440,8 +506,10 @@ __device__ double interp_weno5(const SRC& src, double cx, double cy, double cz, 
   return w0 * S0 + w1 * S1 + w2 * S2;
 }
 
-// propagate<4> (steps.cu:986-1036): tricubic Lagrange on floor(x)-1 .. floor(x)+2
-template <int LAT, class SRC>
+// propagate<4> (steps.cu:986-1036): tricubic Lagrange on floor(x)-1 .. floor(x)+2.
+// MIR: mirror boundaries on the axes of g.mirror (steps.cu:1004-1022): nodes beyond a mirrored
+// face are the mirror images of the cells inside.
+template <int LAT, bool MIR, class SRC>
 __device__ double interp_lagr4(const SRC& src, double cx, double cy, double cz, const typename Lat<LAT>::Tgt& t,
                                double sqTb) {
   const Grid& g = src.g;
@@ -453,21 +521,35 @@ __device__ double interp_lagr4(const SRC& src, double cx, double cy, double cz, 
   lag4_factors(cy - ly, Fy);
   lag4_factors(cz - lz, Fz);
   int yy[4], zz[4];
+  bool ry[4] = {false, false, false, false}, rz[4] = {false, false, false, false};
 #pragma unroll
-  for (int a = 0; a < 4; a++) { yy[a] = wrapi(ly + a, g.ny); zz[a] = wrapi(lz + a, g.nz); }
+  for (int a = 0; a < 4; a++) {
+    if (MIR) {
+      yy[a] = mirror_wrap(ly + a, g.ny, (g.mirror & 2) != 0, ry[a]);
+      zz[a] = mirror_wrap(lz + a, g.nz, (g.mirror & 4) != 0, rz[a]);
+    } else {
+      yy[a] = wrapi(ly + a, g.ny);
+      zz[a] = wrapi(lz + a, g.nz);
+    }
+  }
   const bool line = (g.ny == 1) && (g.nz == 1);
   double fpr = 0;
   for (int ix = 0; ix < 4; ix++) {
-    const int x = wrapi(lx + ix, g.nx);
+    bool rx = false;
+    const int x = MIR ? mirror_wrap(lx + ix, g.nx, (g.mirror & 1) != 0, rx) : wrapi(lx + ix, g.nx);
     const double p = 1 * Fx[ix][0] * Fx[ix][1] * Fx[ix][2];
     double v1 = 0;
-    if (line) v1 = Lat<LAT>::xfer(src, (uint32_t)x * 8u, t, sqTb);
+    if (line && !MIR) v1 = Lat<LAT>::xfer(src, (uint32_t)x * 8u, t, sqTb);
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
       for (int b = 0; b < 4; b++) {
         double T;
-        if (line) {
+        if (MIR) {
+          const uint32_t off = node_off(g, x, yy[a], zz[b]);
+          T = (rx || ry[a] || rz[b]) ? Lat<LAT>::xferm(src, off, t, sqTb, rx, ry[a], rz[b])
+                                     : Lat<LAT>::xfer(src, off, t, sqTb);
+        } else if (line) {
           T = v1;
         } else {
           T = Lat<LAT>::xfer(src, node_off(g, x, yy[a], zz[b]), t, sqTb);
@@ -509,7 +591,9 @@ __device__ __forceinline__ bool f64_once(const SRC& src, const NodeSrc& gsrc, do
       const double vz = s * Lat<LAT>::efc(i, 2) + uT[2];
       const typename Lat<LAT>::Tgt t = Lat<LAT>::tgt(i, uT);
       const double cx = (double)x - vx, cy = (double)y - vy, cz = (double)z - vz;
-      const double fn = INTERP == 1   ? interp_lagr4<LAT>(src, cx, cy, cz, t, a.sqrt_tbase)
+      // INTERP 3: Lagrange-4 with mirror boundaries
+      const double fn = INTERP == 1   ? interp_lagr4<LAT, false>(src, cx, cy, cz, t, a.sqrt_tbase)
+                        : INTERP == 3 ? interp_lagr4<LAT, true>(src, cx, cy, cz, t, a.sqrt_tbase)
                         : INTERP == 2 ? interp_weno5<LAT, true>(src, cx, cy, cz, t, a.sqrt_tbase)
                                       : interp_weno5<LAT, false>(src, cx, cy, cz, t, a.sqrt_tbase);
       rho += fn;
@@ -727,10 +811,21 @@ __global__ __launch_bounds__(128) void k_step_q125_f64(const double* __restrict_
   const int y = (int)((cell / g.nx) % g.ny);
   const int z = (int)(cell / ((int64_t)g.nx * g.ny));
   for (int n = 0; n < 27; n++) {
-    const int px = wrapi(x - 1 + n / 9, g.nx), py = wrapi(y - 1 + (n / 3) % 3, g.ny), pz = wrapi(z - 1 + n % 3, g.nz);
+    // mirror boundaries (steps.cu:163-190): a node beyond a mirrored face is the mirror image of
+    // the cell inside, population k read from the mirrored velocity, the axis velocity negated
+    bool rx, ry, rz;
+    const int px = mirror_wrap(x - 1 + n / 9, g.nx, (g.mirror & 1) != 0, rx);
+    const int py = mirror_wrap(y - 1 + (n / 3) % 3, g.ny, (g.mirror & 2) != 0, ry);
+    const int pz = mirror_wrap(z - 1 + n % 3, g.nz, (g.mirror & 4) != 0, rz);
     const int64_t c = cell_index(g, px, py, pz);
-    if (lane_ok) sf[n][k] = fin[(int64_t)k * P + c];
-    if (k < 4) sg[n][k] = min[(int64_t)(k + 1) * P + c];
+    const int kq = lane_ok ? k : 0;  // lanes 125..127 hold no population
+    const int kx = kq / 25, ky = (kq / 5) % 5, kz = kq % 5;
+    const int kl = 25 * (rx ? kMir5[kx] : kx) + 5 * (ry ? kMir5[ky] : ky) + (rz ? kMir5[kz] : kz);
+    if (lane_ok) sf[n][k] = fin[(int64_t)kl * P + c];
+    if (k < 4) {
+      const double v = min[(int64_t)(k + 1) * P + c];
+      sg[n][k] = (k == 0 && rx) || (k == 1 && ry) || (k == 2 && rz) ? -v : v;
+    }
     if (k == 4) sok[n] = okin[c];
   }
   double rho0 = min[cell], c0[4] = {min[P + cell], min[2 * P + cell], min[3 * P + cell], min[4 * P + cell]};
@@ -937,23 +1032,30 @@ hipError_t launch_step_f64(int lattice, int interp, const double* fin, const dou
     hipLaunchKernelGGL((k_step_f64_rest<L, I, true>), dim3(blocks2), dim3(64), 0, st, src, fout, mout, niter,    \
                        pend, a);                                                                                 \
   } while (0)
+  // Lagrange-4 with mirror boundaries is its own instantiation (3), so the periodic kernels
+  // carry none of the mirror code
+  if (interp == 1 && a.g.mirror) interp = 3;
   if (lattice == kD3Q27 && buf && a.refill_chunk > 0) {
     if (interp == 0) POND_LAUNCH_R(kD3Q27, 0);
     else if (interp == 1) POND_LAUNCH_R(kD3Q27, 1);
+    else if (interp == 3) POND_LAUNCH_R(kD3Q27, 3);
     else POND_LAUNCH_R(kD3Q27, 2);
   } else if (lattice == kD3Q27) {
     if (buf) {
       if (interp == 0) POND_LAUNCH(kD3Q27, 0, true);
       else if (interp == 1) POND_LAUNCH(kD3Q27, 1, true);
+      else if (interp == 3) POND_LAUNCH(kD3Q27, 3, true);
       else POND_LAUNCH(kD3Q27, 2, true);
     } else {
       if (interp == 0) POND_LAUNCH(kD3Q27, 0, false);
       else if (interp == 1) POND_LAUNCH(kD3Q27, 1, false);
+      else if (interp == 3) POND_LAUNCH(kD3Q27, 3, false);
       else POND_LAUNCH(kD3Q27, 2, false);
     }
   } else {
     // on a line (Ny = Nz = 1) the centred and as-written WENO5 coincide
     if (interp == 1) POND_LAUNCH(kD1Q5, 1, false);
+    else if (interp == 3) POND_LAUNCH(kD1Q5, 3, false);
     else POND_LAUNCH(kD1Q5, 0, false);
   }
 #undef POND_LAUNCH

@@ -174,6 +174,27 @@ __device__ __forceinline__ float node_xfer(const NodeSrcF& s, int64_t c, const T
   return transfer27f(f, n.ux, n.uy, n.uz, n.sq, t, sqTb);
 }
 
+// node_xfer of the node's mirror image on the axes set in mx, my, mz (Cell::mirrX/Y/Z,
+// structs.cuh:552-578): the axis velocity negated, every population exchanged with its mirror
+// image
+__device__ float node_xfer_mirrored(const NodeSrcF& s, int64_t c, const TgtF& t, float sqTb, bool mx, bool my,
+                                    bool mz) {
+  const uint32_t boff = (uint32_t)c * 4u;
+  float f0[27], f[27];
+  load27(s.f, boff, f0);
+  const Gauge4 n = ldg4(s, boff);
+#pragma unroll
+  for (int i = 0; i < 27; i++) {
+    const int jx = kE27[i][0] + 1, jy = kE27[i][1] + 1, jz = kE27[i][2] + 1;
+    const float a0 = mx ? f0[kIdx27.v[2 - jx][jy][jz]] : f0[i];
+    const float a1 = mx ? f0[kIdx27.v[2 - jx][2 - jy][jz]] : f0[kIdx27.v[jx][2 - jy][jz]];
+    const float a2 = mx ? f0[kIdx27.v[2 - jx][jy][2 - jz]] : f0[kIdx27.v[jx][jy][2 - jz]];
+    const float a3 = mx ? f0[kIdx27.v[2 - jx][2 - jy][2 - jz]] : f0[kIdx27.v[jx][2 - jy][2 - jz]];
+    f[i] = mz ? (my ? a3 : a2) : (my ? a1 : a0);
+  }
+  return transfer27f(f, mx ? -n.ux : n.ux, my ? -n.uy : n.uy, mz ? -n.uz : n.uz, n.sq, t, sqTb);
+}
+
 __device__ __forceinline__ void lag3f(float s, float F[3]) {
   // quadratic Lagrange basis on nodes 0,1,2 at s
   F[0] = 0.5f * (s - 1.0f) * (s - 2.0f);
@@ -261,6 +282,8 @@ __device__ float interp_weno5f(const NodeSrcF& src, double cx, double cy, double
   return weno5_combine((float)(cx - r), ub, Sv);
 }
 
+// MIR: mirror boundaries on the axes of g.mirror (steps.cu:1004-1022)
+template <bool MIR>
 __device__ float interp_lagr4f(const NodeSrcF& src, double cx, double cy, double cz, const TgtF& t, float sqTb) {
   const Grid& g = src.g;
   const int lx = floor_int(cx) - 1, ly = floor_int(cy) - 1, lz = floor_int(cz) - 1;
@@ -269,17 +292,32 @@ __device__ float interp_lagr4f(const NodeSrcF& src, double cx, double cy, double
   lag4f((float)(cy - ly), Fy);
   lag4f((float)(cz - lz), Fz);
   int yy[4], zz[4];
+  bool ry[4] = {false, false, false, false}, rz[4] = {false, false, false, false};
 #pragma unroll
-  for (int a = 0; a < 4; a++) { yy[a] = wrapi(ly + a, g.ny); zz[a] = wrapi(lz + a, g.nz); }
+  for (int a = 0; a < 4; a++) {
+    if (MIR) {
+      yy[a] = mirror_wrap(ly + a, g.ny, (g.mirror & 2) != 0, ry[a]);
+      zz[a] = mirror_wrap(lz + a, g.nz, (g.mirror & 4) != 0, rz[a]);
+    } else {
+      yy[a] = wrapi(ly + a, g.ny);
+      zz[a] = wrapi(lz + a, g.nz);
+    }
+  }
   float out = 0.f;
   for (int ix = 0; ix < 4; ix++) {
-    const int x = wrapi(lx + ix, g.nx);
+    bool rx = false;
+    const int x = MIR ? mirror_wrap(lx + ix, g.nx, (g.mirror & 1) != 0, rx) : wrapi(lx + ix, g.nx);
     float ys = 0.f;
 #pragma unroll
     for (int a = 0; a < 4; a++) {
       float zs = 0.f;
 #pragma unroll
-      for (int b = 0; b < 4; b++) zs = __builtin_fmaf(Fz[b], node_xfer(src, cell_index(g, x, yy[a], zz[b]), t, sqTb), zs);
+      for (int b = 0; b < 4; b++) {
+        const int64_t c = cell_index(g, x, yy[a], zz[b]);
+        const float v = MIR && (rx || ry[a] || rz[b]) ? node_xfer_mirrored(src, c, t, sqTb, rx, ry[a], rz[b])
+                                                      : node_xfer(src, c, t, sqTb);
+        zs = __builtin_fmaf(Fz[b], v, zs);
+      }
       ys = __builtin_fmaf(Fy[a], zs, ys);
     }
     out = __builtin_fmaf(Fx[ix], ys, out);
@@ -579,7 +617,10 @@ __device__ __forceinline__ bool split27_once(const NodeSrcF& src, float* __restr
         const double vz = s * (double)ez + gc[2];
         const TgtF t = make_tgtf(i, gc);
         const double cx = (double)x - vx, cy = (double)y - vy, cz = (double)z - vz;
-        const float fn = INTERP == 1 ? interp_lagr4f(src, cx, cy, cz, t, sqTb) : interp_weno5f<false>(src, cx, cy, cz, t, sqTb);
+        // INTERP 3: Lagrange-4 with mirror boundaries
+        const float fn = INTERP == 1   ? interp_lagr4f<false>(src, cx, cy, cz, t, sqTb)
+                         : INTERP == 3 ? interp_lagr4f<true>(src, cx, cy, cz, t, sqTb)
+                                       : interp_weno5f<false>(src, cx, cy, cz, t, sqTb);
         add(ex, ey, ez, fn);
       }
     }
@@ -678,11 +719,22 @@ __device__ __forceinline__ void split27_finish(float* __restrict__ fout, float* 
   niter_out[cell] = st.niter;
 }
 
+// A predicted start is taken only in cells whose previous step converged within this many
+// iterations.  Where the gauge map contracts slowly (cells that needed more, on shocks and
+// contact layers) it can have a second fixed point near the first, and a start from elsewhere
+// may settle there: on the 384^3 shock-bubble, ungated predicted starts left 13-189 cells per
+// step 1e-4 to 2e-3 (relative, u and T) away from the FP64 step taken from the same state, all
+// in cells whose FP64 step needed 9-10 iterations; gated at 2, 3, 4 or 6 previous iterations the
+// one-step error stayed at the plain start's 2e-7 on every cell.
+constexpr int kLearnedPrevIters = 4;
+
 // Starting state of a cell: its previous gauge, or the predicted one where a prediction is
-// given, finite, not all zero, has T > 0 and lies within the trust radius.
+// given, finite, not all zero, has T > 0, lies within the trust radius and the cell's previous
+// step took at most kLearnedPrevIters iterations (niter_prev: the counts of the previous step,
+// which this step overwrites cell by cell after reading).
 __device__ __forceinline__ void split27_start(const float* __restrict__ min, const double* __restrict__ Tin,
-                                              const float* __restrict__ guess, int64_t cell, double gp[4],
-                                              SplitIter& st, const SplitArgs& a) {
+                                              const float* __restrict__ guess, const int* __restrict__ niter_prev,
+                                              int64_t cell, double gp[4], SplitIter& st, const SplitArgs& a) {
   const int64_t P = a.g.plane;
   double rho0;
   split_prev_gauge(min, Tin, P, cell, a.tlat, gp, rho0);
@@ -701,7 +753,7 @@ __device__ __forceinline__ void split27_start(const float* __restrict__ min, con
     const float pmax = fmaxf(fmaxf(fabsf(p0), fabsf(p1)), fmaxf(fabsf(p2), fabsf(p3)));
     // (a prediction of no change at all is the previous-gauge start, with its stopping rules)
     if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0 && pmax <= a.learned_trust &&
-        pmax > 0.0f) {
+        pmax > 0.0f && niter_prev[cell] <= kLearnedPrevIters) {
       for (int k = 0; k < 4; k++) st.gc[k] = q[k];
       st.learned = true;
     }
@@ -730,7 +782,7 @@ __device__ __forceinline__ void split27_pass1(const float* __restrict__ fin, con
   const int64_t cell = live ? cell_index(g, x, y, z) : 0;
   if (live) {
     double gp[4];
-    split27_start(min, Tin, guess, cell, gp, st, a);
+    split27_start(min, Tin, guess, niter_out, cell, gp, st, a);
     const NodeSrcF src = make_src(fin, g4in, g);
     const bool conv = split27_iterate<INTERP>(src, fout, cell, x, y, z, gp, st, a.pass1_iters, a);
     pending = !conv && st.niter - st.base < a.max_iter;
@@ -793,7 +845,7 @@ __device__ __forceinline__ void split27_refill(const float* __restrict__ fin, co
       const int64_t slot = next + rank;
       if (slot < total && tile_cell(g, t0 + slot / 64, (int)(slot % 64), x, y, z)) {
         cell = cell_index(g, x, y, z);
-        split27_start(min, Tin, guess, cell, gp, st, a);
+        split27_start(min, Tin, guess, niter_out, cell, gp, st, a);
         active = true;
       }
     }
@@ -901,17 +953,26 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
   const int y = (int)((cell / g.nx) % g.ny);
   const int z = (int)(cell / ((int64_t)g.nx * g.ny));
   for (int n = 0; n < 27; n++) {
-    const int px = wrapi(x - 1 + n / 9, g.nx), py = wrapi(y - 1 + (n / 3) % 3, g.ny), pz = wrapi(z - 1 + n % 3, g.nz);
+    // mirror boundaries (steps.cu:163-190): a node beyond a mirrored face is the mirror image of
+    // the cell inside, population k read from the mirrored velocity, the axis velocity negated
+    bool rx, ry, rz;
+    const int px = mirror_wrap(x - 1 + n / 9, g.nx, (g.mirror & 1) != 0, rx);
+    const int py = mirror_wrap(y - 1 + (n / 3) % 3, g.ny, (g.mirror & 2) != 0, ry);
+    const int pz = mirror_wrap(z - 1 + n % 3, g.nz, (g.mirror & 4) != 0, rz);
     const int64_t c = cell_index(g, px, py, pz);
+    const int kq = lane_ok ? k : 0;  // lanes 125..127 hold no population
+    const int kx = kq / 25, ky = (kq / 5) % 5, kz = kq % 5;
+    const int kl = 25 * (rx ? kMir5[kx] : kx) + 5 * (ry ? kMir5[ky] : ky) + (rz ? kMir5[kz] : kz);
     // non-finite data as in k_node_gauge (the reference drops non-finite transfer terms,
     // structs.cuh:516): a non-finite population stages as zero, a node with a non-finite gauge
     // as all zeros with a zero gauge
-    const float ux = min[P + c], uy = min[2 * P + c], uz = min[3 * P + c];
+    const float ux = rx ? -min[P + c] : min[P + c], uy = ry ? -min[2 * P + c] : min[2 * P + c],
+                uz = rz ? -min[3 * P + c] : min[3 * P + c];
     const double t = Tin[c];
     const float s0 = (float)(sqrt(fabs(t)) * (double)((0.0 < t) - (t < 0.0)));
     const bool gok = isfinite(ux) && isfinite(uy) && isfinite(uz) && isfinite(s0);
     if (lane_ok) {
-      const float v = fin[(int64_t)k * P + c];
+      const float v = fin[(int64_t)kl * P + c];
       sf[n][k] = gok && isfinite(v) ? v : 0.0f;
     }
     if (k < 3) su[n][k] = gok ? (k == 0 ? ux : k == 1 ? uy : uz) : 0.0f;
@@ -929,7 +990,7 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
     const float pmax = fmaxf(fmaxf(fabsf(guess[cell]), fabsf(guess[P + cell])),
                              fmaxf(fabsf(guess[2 * P + cell]), fabsf(guess[3 * P + cell])));
     if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0 && pmax <= a.learned_trust &&
-        pmax > 0.0f) {
+        pmax > 0.0f && niter_out[cell] <= kLearnedPrevIters) {
       for (int d = 0; d < 4; d++) gc[d] = q[d];
       learned = true;
     }
@@ -1110,7 +1171,8 @@ hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const
                        out.m, out.T, niter, guess, a, a.refill_chunk);
     return hipGetLastError();
   }
-  if (interp == 1) POND_LAUNCH_S(k_step_split27<1>, 1);
+  if (interp == 1 && a.g.mirror) POND_LAUNCH_S(k_step_split27<3>, 3);
+  else if (interp == 1) POND_LAUNCH_S(k_step_split27<1>, 1);
   else if (interp == 2) POND_LAUNCH_S(k_step_split27<2>, 2);
   else POND_LAUNCH_S(k_step_split27<0>, 0);
 #undef POND_LAUNCH_S

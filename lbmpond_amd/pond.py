@@ -59,6 +59,7 @@ class PondParams(ctypes.Structure):
         ("pass1_iters", ctypes.c_int),
         ("global_loads", ctypes.c_int),
         ("learned_trust", ctypes.c_double),
+        ("mirror", ctypes.c_int),
     ]
 
 
@@ -71,6 +72,7 @@ class PondDiag(ctypes.Structure):
         ("niter_max", ctypes.c_int32),
         ("niter_mean", ctypes.c_double),
         ("step", ctypes.c_int64),
+        ("max_departure", ctypes.c_double),
     ]
 
 
@@ -100,6 +102,7 @@ SIGNATURES = {
     "pond_lattice_tables": (ctypes.c_int, [ctypes.c_int, _I32, _D, _D, _D]),
     "pond_set_state": (ctypes.c_int, [_CTX, _D, _D, _D, _D, _D]),
     "pond_set_populations": (ctypes.c_int, [_CTX, _D, _D, _D, _D, _D, _D]),
+    "pond_copy_state": (ctypes.c_int, [_CTX, _CTX]),
     "pond_step": (ctypes.c_int, [_CTX, ctypes.c_int]),
     "pond_sync": (ctypes.c_int, [_CTX]),
     "pond_get_fields": (ctypes.c_int, [_CTX, _D, _D, _D, _D, _D, _I32]),
@@ -188,10 +191,11 @@ class Pond:
     def __init__(self, lattice="D3Q27", shape=(16, 16, 16), interp="weno5", precision="fp64", dtau=None,
                  tol_abs=None, tol_rel=None, max_iter=None, use_mlp=False, split_tol_rel=None,
                  split_tol_abs=None, tile_patch=None, refill_chunk=None, pass1_iters=None, global_loads=None,
-                 learned_trust=None):
+                 learned_trust=None, mirror=None):
         """Parameters left at None keep pond_params_default's value.  tile_patch, refill_chunk,
         pass1_iters and global_loads set the D3Q27 kernels' work distribution (no effect on
-        results); learned_trust the learned start's trust radius (include/pond.h)."""
+        results); learned_trust the learned start's trust radius; mirror the mirror-boundary axes,
+        bit d for axis d (include/pond.h)."""
         L = lib()
         p = PondParams()
         L.pond_params_default(ctypes.byref(p), LATTICES[lattice])
@@ -216,6 +220,8 @@ class Pond:
                 setattr(p, name, int(val))
         if learned_trust is not None:
             p.learned_trust = float(learned_trust)
+        if mirror is not None:
+            p.mirror = int(mirror)
         p.use_mlp = 1 if use_mlp else 0
         h = ctypes.c_void_p()
         st = L.pond_create(ctypes.byref(p), ctypes.byref(h))
@@ -261,6 +267,11 @@ class Pond:
         a = [_c64(v, self.n) for v in (rho, ux, uy, uz, T)]
         self._check(lib().pond_set_populations(self._h, _dptr(fa), *[_dptr(v) for v in a]),
                     "pond_set_populations")
+
+    def copy_state_from(self, other):
+        """the current state of another context of the same lattice and grid (pond_copy_state),
+        converted to this context's precision on the device"""
+        self._check(lib().pond_copy_state(self._h, other._h), "pond_copy_state")
 
     def step(self, n=1):
         self._check(lib().pond_step(self._h, int(n)), "pond_step")

@@ -251,3 +251,42 @@ def test_checkpoint_restart_is_exact(gpu, tmp_path, precision):
     with Pond("D3Q27", (20, 9, 8), "weno5c", precision) as r:
         with pytest.raises(Exception):
             r.restore(path)
+
+
+# ---- the work distribution the large grids run: 4x4 (y, z) row patches of 64-cell row segments
+# (used where nx % 64 == 0 and 4 divides ny and nz) with the default lane-refill chunks
+
+
+def _row_patch_grid(shape):
+    nx, ny, nz = shape
+    return nx % 64 == 0 and ny % 4 == 0 and nz % 4 == 0
+
+
+@pytest.mark.parametrize("shape,interp,steps", [((64, 8, 8), "weno5c", 3), ((128, 8, 4), "lagr4", 2)])
+def test_fp64_row_patch_order_matches_oracle(gpu, shape, interp, steps):
+    """FP64 on a grid that takes the row-patch order and the default refill chunks and first-pass
+    length (the configuration of every 256^3 and 384^3 run) equals the oracle bit for bit, and so
+    does plain row order with one tile per wave."""
+    assert _row_patch_grid(shape)
+    f = smooth_random_state(shape, seed=81, T0=0.01)
+    o = po.Oracle("D3Q27", shape, interp)
+    o.set_state(*f)
+    o.step(steps)
+    ref = o.fields()
+    for kw in ({}, {"tile_patch": 0, "refill_chunk": 0}):
+        print(f"[row patch f64] {shape} {interp} {kw}", flush=True)
+        with Pond("D3Q27", shape, interp, "fp64", **kw) as p:
+            p.set_state(*f)
+            p.step(steps)
+            got = p.fields()
+        for key in FIELDS + ("niter",):
+            assert np.array_equal(got[key], ref[key]), (kw, key)
+
+
+def test_d3q125_100_steps(gpu):
+    """D3Q125 (PonDstepD3Q125, steps.cu:151-250) against the oracle for 100 steps."""
+    shape = (4, 4, 4)
+    f = smooth_random_state(shape, seed=33, T0=0.02)
+    ref, got, fr, fg = run_pair("D3Q125", shape, "weno5", f, 100)
+    assert np.isfinite(ref["rho"]).all()
+    check(ref, got, fr, fg)

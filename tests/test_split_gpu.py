@@ -230,3 +230,73 @@ def test_split_nonfinite_nodes_match_fp64(gpu, lattice, interp, shape):
     print("  split vs fp64 on the finite cells:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
     for k in STATE_ERRORS:
         assert errs[k] <= 1e-5, (k, errs[k])
+
+
+@pytest.mark.parametrize("learned", [False, True])
+def test_split_row_patch_order_is_bitwise_neutral(gpu, learned):
+    """The split step's row-patch order with whole-row refill chunks (the default wherever
+    nx % 64 == 0 and 4 divides ny and nz: every 256^3 and 384^3 run) gives the same bits as plain
+    row order with one tile per wave, with the plain and the learned start."""
+    shape = (128, 8, 8)
+    f = smooth_random_state(shape, seed=82, T0=0.01)
+    outs = []
+    for kw in ({}, {"tile_patch": 0, "refill_chunk": 0}, {"tile_patch": 0}):
+        print(f"[row patch split] learned={learned} {kw}", flush=True)
+        with Pond("D3Q27", shape, "weno5c", "split", use_mlp=learned, **kw) as p:
+            if learned:
+                p.mlp_init(seed=5)
+                params, scales = p.mlp_params()
+                p.mlp_set_params(0.2 * params, scales)
+            p.set_state(*f)
+            p.step(3)
+            outs.append(p.fields())
+    for o in outs[1:]:
+        for k in ("rho", "ux", "uy", "uz", "T", "niter"):
+            assert np.array_equal(o[k], outs[0][k]), k
+
+
+def test_split_against_oracle(gpu):
+    """The split step against the CPU oracle itself (not through the FP64 GPU path): 10 steps
+    of a smooth random state, relative L-inf <= 1e-5 on rho, u and T."""
+    import pyoracle as po
+    shape = (16, 12, 10)
+    f = smooth_random_state(shape, seed=65, T0=0.01)
+    o = po.Oracle("D3Q27", shape, "weno5c")
+    o.set_state(*f)
+    o.step(10)
+    ref = o.fields()
+    with Pond("D3Q27", shape, "weno5c", "split") as p:
+        p.set_state(*f)
+        p.step(10)
+        got = p.fields()
+    errs = state_errors(got, ref)
+    print("  split vs oracle rel Linf:", {k: f"{v:.2e}" for k, v in errs.items()},
+          f" mean iters oracle {ref['niter'].mean():.2f} split {got['niter'].mean():.2f}", flush=True)
+    assert (ref["niter"] < 100).all() and (got["niter"] < 100).all()
+    for k in STATE_ERRORS:
+        assert errs[k] <= 1e-5, (k, errs[k])
+
+
+def test_copy_state_between_precisions(gpu):
+    """pond_copy_state: an FP64 state copied into a split context steps exactly like the same
+    state uploaded with pond_set_populations (FP64 -> FP32 rounding to nearest either way), and
+    an FP64 -> FP64 copy continues bit for bit like the source"""
+    shape = (16, 12, 10)
+    f = smooth_random_state(shape, seed=66, T0=0.01)
+    with Pond("D3Q27", shape, "weno5c", "fp64") as a:
+        a.set_state(*f)
+        a.step(2)
+        pop = a.populations()
+        fl = a.fields()
+        with Pond("D3Q27", shape, "weno5c", "split") as b, Pond("D3Q27", shape, "weno5c", "split") as c, \
+                Pond("D3Q27", shape, "weno5c", "fp64") as d:
+            b.copy_state_from(a)
+            c.set_populations(pop, fl["rho"], fl["ux"], fl["uy"], fl["uz"], fl["T"])
+            d.copy_state_from(a)
+            assert b.diagnostics()["step"] == 2
+            for p in (a, b, c, d):
+                p.step(2)
+            gb, gc, ga, gd = b.fields(), c.fields(), a.fields(), d.fields()
+    for k in ("rho", "ux", "uy", "uz", "T", "niter"):
+        assert np.array_equal(gb[k], gc[k]), k
+        assert np.array_equal(ga[k], gd[k]), k
