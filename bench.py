@@ -100,7 +100,7 @@ def kernel_name(lattice, prec, interp):
     code = {"weno5": 0, "lagr4": 1, "weno5c": 2}[interp]
     if prec == "fp64":
         return f"k_step_f64_refill<1,{code},true>"
-    return "k_step_split27_refill<2>" if interp == "weno5c" else f"k_step_split27<{code}>"
+    return f"k_step_split27_refill<{code}>" if code != 0 else "k_step_split27<0>"
 
 
 class Run:

@@ -29,6 +29,10 @@ struct MlpState {
   Grid g;
   const float* m;   // rho, ux, uy, uz planes
   const double* T;  // temperature plane
+  // when set, only 16-cell tiles holding a cell whose count lies in [gate_lo, gate_hi] are
+  // evaluated (the step takes a predicted start nowhere else); the others' outputs are not written
+  const int* niter = nullptr;
+  int gate_lo = 0, gate_hi = 0;
 };
 
 hipError_t mlp_launch_infer(const MlpWeightsBF& w, const MlpState& s, float* out, int64_t out_plane, hipStream_t st);

@@ -157,11 +157,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const int64_t nunit = MODE == 0 ? (int64_t)nseg * g.ny * g.nz : (ncell + 15) / 16;
   for (int64_t unit = wave; unit < nunit; unit += nwave) {
     int xs0 = 0, y = 0, z = 0;
+    uint64_t gate = ~0ull;  // 16-cell tiles to evaluate, one bit per cell
     if (MODE == 0) {
       xs0 = 64 * (int)(unit % nseg);
       const int64_t row = unit / nseg;
       y = (int)(row % g.ny);
       z = (int)(row / g.ny);
+      if (s.niter) {
+        // cells of the segment that will take a predicted start (one per lane, uniform result)
+        const int xl = xs0 + l;
+        bool want = false;
+        if (xl < g.nx) {
+          const int ni = s.niter[cell_index(g, xl, y, z)];
+          want = ni >= s.gate_lo && ni <= s.gate_hi;
+        }
+        gate = __ballot(want);
+        if (gate == 0) continue;
+      }
       wave_sync();  // the previous segment's reads of this wave's LDS are done
       // Entries e = l + 64 i.  The loop is unrolled and its indices wrap without branches
       // (wrap2: one period either way, which the y and z rows always are and the x positions
@@ -187,6 +199,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
 #pragma unroll 1
   for (int sub = 0; sub < (MODE == 0 ? 4 : 1); sub++) {
+    if (MODE == 0 && ((gate >> (16 * sub)) & 0xFFFFull) == 0) continue;
     int64_t cell;
     bool live;
     // B operands of layer 1: features 16*st + 4*q + e of this lane's cell

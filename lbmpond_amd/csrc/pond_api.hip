@@ -643,8 +643,10 @@ pond_status pond_step(pond_ctx* ctx, int nsteps) {
     if (learned) {
       // predicted starting gauge from the load state (part of the step's work)
       POND_HIP(ctx, hipEventRecord(ctx->mlp_ev[0], ctx->stream));
-      he = mlp_launch_infer(pond_mlp_weights(ctx), MlpState{ctx->g, ctx->sp[in].m, ctx->sp[in].T}, ctx->guess,
-                            ctx->g.plane, ctx->stream);
+      // only the tiles holding a cell that will take the prediction (kLearnedPrevMin ..
+      // kLearnedPrevMax iterations in the previous step) are evaluated
+      MlpState ms{ctx->g, ctx->sp[in].m, ctx->sp[in].T, ctx->niter, kLearnedPrevMin, kLearnedPrevMax};
+      he = mlp_launch_infer(pond_mlp_weights(ctx), ms, ctx->guess, ctx->g.plane, ctx->stream);
       if (he != hipSuccess) return fail(ctx, POND_ERR_HIP, std::string("mlp inference: ") + hipGetErrorString(he));
       POND_HIP(ctx, hipEventRecord(ctx->mlp_ev[1], ctx->stream));
       ctx->mlp_pending = true;

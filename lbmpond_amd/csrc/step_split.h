@@ -19,6 +19,10 @@ struct SplitArgs {
   float w[kQmax];
 };
 
+// The predicted (learned) start is taken only in cells whose previous step took
+// kLearnedPrevMin .. kLearnedPrevMax gauge iterations (step_split.hip gives the measurements).
+constexpr int kLearnedPrevMin = 3, kLearnedPrevMax = 4;
+
 // D3Q27 split populations are stored in seven quad planes: population k of cell c at float index
 // ((k / 4) * plane + c) * 4 + k % 4 (16 bytes per cell and quad plane, the 28th slot unused), so
 // a stencil node's 27 populations are seven 16-byte loads.  D3Q125 keeps one plane per

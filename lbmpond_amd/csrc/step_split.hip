@@ -175,24 +175,42 @@ __device__ __forceinline__ float node_xfer(const NodeSrcF& s, int64_t c, const T
 }
 
 // node_xfer of the node's mirror image on the axes set in mx, my, mz (Cell::mirrX/Y/Z,
-// structs.cuh:552-578): the axis velocity negated, every population exchanged with its mirror
-// image
+// structs.cuh:552-578: the axis velocity negated, every population exchanged with its mirror
+// image).  Exchanging the populations along an axis is the same as reversing that axis's basis
+// values (sum_j L[j] f[rev j] = sum_j L[rev j] f[j]), so the populations stay in place and only
+// the three basis values of a mirrored axis are swapped end for end.
 __device__ float node_xfer_mirrored(const NodeSrcF& s, int64_t c, const TgtF& t, float sqTb, bool mx, bool my,
                                     bool mz) {
   const uint32_t boff = (uint32_t)c * 4u;
-  float f0[27], f[27];
-  load27(s.f, boff, f0);
+  float f[27];
+  load27(s.f, boff, f);
   const Gauge4 n = ldg4(s, boff);
+  float Lx[3], Ly[3], Lz[3];
+  basis3(((t.u1x - (mx ? -n.ux : n.ux)) + t.u1xl) * sqTb, n.sq, t.kx, t.cx, t.T1, t.wx, Lx);
+  basis3(((t.u1y - (my ? -n.uy : n.uy)) + t.u1yl) * sqTb, n.sq, t.ky, t.cy, t.T1, t.wy, Ly);
+  basis3(((t.u1z - (mz ? -n.uz : n.uz)) + t.u1zl) * sqTb, n.sq, t.kz, t.cz, t.T1, t.wz, Lz);
+  auto flip = [](bool m, float L[3]) {
+    const float a = L[0];
+    L[0] = m ? L[2] : L[0];
+    L[2] = m ? a : L[2];
+  };
+  flip(mx, Lx);
+  flip(my, Ly);
+  flip(mz, Lz);
+  float out = 0.0f;
 #pragma unroll
-  for (int i = 0; i < 27; i++) {
-    const int jx = kE27[i][0] + 1, jy = kE27[i][1] + 1, jz = kE27[i][2] + 1;
-    const float a0 = mx ? f0[kIdx27.v[2 - jx][jy][jz]] : f0[i];
-    const float a1 = mx ? f0[kIdx27.v[2 - jx][2 - jy][jz]] : f0[kIdx27.v[jx][2 - jy][jz]];
-    const float a2 = mx ? f0[kIdx27.v[2 - jx][jy][2 - jz]] : f0[kIdx27.v[jx][jy][2 - jz]];
-    const float a3 = mx ? f0[kIdx27.v[2 - jx][2 - jy][2 - jz]] : f0[kIdx27.v[jx][2 - jy][2 - jz]];
-    f[i] = mz ? (my ? a3 : a2) : (my ? a1 : a0);
+  for (int a = 0; a < 3; a++) {
+    float sy = 0.0f;
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      float sz = 0.0f;
+#pragma unroll
+      for (int cc = 0; cc < 3; cc++) sz = __builtin_fmaf(Lz[cc], f[kIdx27.v[a][b][cc]], sz);
+      sy = __builtin_fmaf(Ly[b], sz, sy);
+    }
+    out = __builtin_fmaf(Lx[a], sy, out);
   }
-  return transfer27f(f, mx ? -n.ux : n.ux, my ? -n.uy : n.uy, mz ? -n.uz : n.uz, n.sq, t, sqTb);
+  return out;
 }
 
 __device__ __forceinline__ void lag3f(float s, float F[3]) {
@@ -617,37 +635,46 @@ __device__ void interp_pass_weno5c(const NodeSrcF& src, int x, int y, int z, dou
             for (int jc = 0; jc < 3; jc++) sz = __builtin_fmaf(Lz[jc], f[kIdx27.v[ja][jb][jc]], sz);
             szz[ja][jb] = sz;
           }
+        // the x basis of the three targets is A_j (A_j - c) or A_j^2 - T1 with A_j = Dx - (j-1) sq0:
+        // the x moments R = sum_ja sz, Q = sum_ja A_ja sz, P = sum_ja A_ja^2 sz of the z sums, per
+        // source y component jb, are common to every group; each group then contracts them
+        // over y and its three targets take P + sq1 Q, P - T1 R and P - sq1 Q
         const float Dy = ((u1y - n.uy) + u1yl) * sqTb;
         const float Dx = ((u1x - n.ux) + u1xl) * sqTb;
         const float A0 = Dx + n.sq, A1 = Dx, A2 = Dx - n.sq;
+        const float B0 = A0 * A0, B1 = A1 * A1, B2 = A2 * A2;
+        float xr[3], xq[3], xp[3];
+#pragma unroll
+        for (int jb = 0; jb < 3; jb++) {
+          xr[jb] = (szz[0][jb] + szz[1][jb]) + szz[2][jb];
+          xq[jb] = __builtin_fmaf(A2, szz[2][jb], __builtin_fmaf(A1, szz[1][jb], A0 * szz[0][jb]));
+          xp[jb] = __builtin_fmaf(B2, szz[2][jb], __builtin_fmaf(B1, szz[1][jb], B0 * szz[0][jb]));
+        }
+        const float Ay0 = Dy + n.sq, Ay1 = Dy, Ay2 = Dy - n.sq;
 #pragma unroll
         for (int e = 0; e < 3; e++) {
           const int a = ya[e];
           if (a < 0 || a > 2) continue;
           const int ey = e - 1;
-          float Ly[3];
-          basis3u(Dy, n.sq, ey ? (float)ey * sq1 : 0.0f, ey ? 0.0f : T1, Ly);
-          float sy[3];
-#pragma unroll
-          for (int ja = 0; ja < 3; ja++) {
-            float acc = 0.f;
-#pragma unroll
-            for (int jb = 0; jb < 3; jb++) acc = __builtin_fmaf(Ly[jb], szz[ja][jb], acc);
-            sy[ja] = acc;
-          }
-          const float Q = __builtin_fmaf(A2, sy[2], __builtin_fmaf(A1, sy[1], A0 * sy[0]));
-          const float Pp = __builtin_fmaf(A2, A2 * sy[2], __builtin_fmaf(A1, A1 * sy[1], A0 * (A0 * sy[0])));
-          const float R = (sy[0] + sy[1]) + sy[2];
-          const float Tt[3] = {__builtin_fmaf(sq1, Q, Pp), __builtin_fmaf(-T1, R, Pp), __builtin_fmaf(-sq1, Q, Pp)};
+          // basis3u of the group: fma(A, A - c, -T) with (c, T) of a moving or resting y axis
+          const float cyb = ey ? (float)ey * sq1 : 0.0f, Tyb = ey ? 0.0f : T1;
+          const float Ly0 = __builtin_fmaf(Ay0, Ay0 - cyb, -Tyb), Ly1 = __builtin_fmaf(Ay1, Ay1 - cyb, -Tyb),
+                      Ly2 = __builtin_fmaf(Ay2, Ay2 - cyb, -Tyb);
+          const float R = __builtin_fmaf(Ly2, xr[2], __builtin_fmaf(Ly1, xr[1], Ly0 * xr[0]));
+          const float Q = __builtin_fmaf(Ly2, xq[2], __builtin_fmaf(Ly1, xq[1], Ly0 * xq[0]));
+          const float Pp = __builtin_fmaf(Ly2, xp[2], __builtin_fmaf(Ly1, xp[1], Ly0 * xp[0]));
           const float fya = a == 0 ? Fy[e][0] : a == 1 ? Fy[e][1] : Fy[e][2];
           const float wab = fya * fzb;
-          const bool is_mid = a == 1 && b == 1;
+          const float Tt[3] = {__builtin_fmaf(sq1, Q, Pp), __builtin_fmaf(-T1, R, Pp), __builtin_fmaf(-sq1, Q, Pp)};
 #pragma unroll
-          for (int t = 0; t < 3; t++) {
-            yz[e][t] = __builtin_fmaf(wab, Tt[t], yz[e][t]);
-            // (selects with constant indices: the arrays stay in registers)
+          for (int t = 0; t < 3; t++) yz[e][t] = __builtin_fmaf(wab, Tt[t], yz[e][t]);
+          // the smoothness samples are the middle node's values: the middle z row (b, uniform
+          // across the wave) and the group's middle y row
+          if (b == 1 && a == 1) {
 #pragma unroll
-            for (int jj = 0; jj < 5; jj++) ub[e][t][jj] = is_mid && jt[t] == jj ? Tt[t] : ub[e][t][jj];
+            for (int t = 0; t < 3; t++)
+#pragma unroll
+              for (int jj = 0; jj < 5; jj++) ub[e][t][jj] = jt[t] == jj ? Tt[t] : ub[e][t][jj];
           }
         }
       }
@@ -742,6 +769,161 @@ constexpr int kLearnedBudget = 12;
 // separate fixed points and a start far from the previous gauge may settle on another one than
 // the reference reaches.
 
+// Lagrange-4 (propagate<4>, steps.cu:986-1036) for the nine targets that share ez, in one pass
+// over the union of their stencil nodes, as interp_pass_weno5c does for WENO5C: the z rows
+// (floor(cz)-1 .. +2) are common, the x windows depend on ex only, and the y rows of the three ey
+// groups differ by at most a row or two.  Each node of the union is loaded once, its z
+// contraction formed once for the three groups; each group whose four rows hold the node
+// contracts over y, and each of its targets whose four x slices hold the node adds the node's
+// value with its tensor Lagrange weight.  The per-axis basis factors w (1/(2T1) or -1/T1) are
+// applied to the finished sums.  MIR: mirror boundaries on the axes of g.mirror (a node beyond
+// a mirrored face is the mirror image of the cell inside).  Windows or rows far apart (absurd
+// gauges) fall back to interp_lagr4f per target.  Each target's value is stored to fout.
+template <bool MIR>
+__device__ void interp_pass_lagr4(const NodeSrcF& src, int x, int y, int z, double s, const double gc[4], int ez,
+                                  float sqTb, float* __restrict__ fout, int64_t cell, int64_t P) {
+  const Grid& g = src.g;
+  const float u1x = (float)gc[0], u1y = (float)gc[1], u1z = (float)gc[2];
+  const float u1xl = (float)(gc[0] - (double)u1x), u1yl = (float)(gc[1] - (double)u1y),
+              u1zl = (float)(gc[2] - (double)u1z);
+  const float T1 = (float)gc[3];
+  const float h = 1.0f / (2.0f * T1), mm = -1.0f / T1;
+  const float sq1 = sqrtf(fabsf(T1)) * (float)((0.0f < T1) - (T1 < 0.0f));
+  const double cz = (double)z - (s * (double)ez + gc[2]);
+  const int lz = floor_int(cz) - 1;
+  float Fz[4];
+  lag4f((float)(cz - lz), Fz);
+  const float czb = ez ? (float)ez * sq1 : 0.0f, Tzb = ez ? 0.0f : T1;
+  const float wz = ez ? h : mm;
+  int ly[3], lx[3];
+  float Fy[3][4], Fx[3][4];
+  double cxv[3], cyv[3];
+#pragma unroll
+  for (int e = 0; e < 3; e++) {
+    cyv[e] = (double)y - (s * (double)(e - 1) + gc[1]);
+    ly[e] = floor_int(cyv[e]) - 1;
+    lag4f((float)(cyv[e] - ly[e]), Fy[e]);
+    cxv[e] = (double)x - (s * (double)(e - 1) + gc[0]);
+    lx[e] = floor_int(cxv[e]) - 1;
+    lag4f((float)(cxv[e] - lx[e]), Fx[e]);
+  }
+  const int xlo = min(lx[0], min(lx[1], lx[2])), xhi = max(lx[0], max(lx[1], lx[2])) + 3;
+  const int ylo = min(ly[0], min(ly[1], ly[2])), yhi = max(ly[0], max(ly[1], ly[2])) + 3;
+  if ((int64_t)xhi - (int64_t)xlo > 6 || (int64_t)yhi - (int64_t)ylo > 6) {
+#pragma unroll 1
+    for (int q = 0; q < 9; q++) {
+      const int e = q / 3, t = q % 3;
+      const int i = kIdx27.v[t][e][ez + 1];
+      fout[split27_fidx(i, cell, P)] = interp_lagr4f<MIR>(src, cxv[t], cyv[e], cz, make_tgtf(i, gc), sqTb);
+    }
+    return;
+  }
+  float acc[3][3];
+#pragma unroll
+  for (int e = 0; e < 3; e++)
+#pragma unroll
+    for (int t = 0; t < 3; t++) acc[e][t] = 0.f;
+#pragma unroll 1
+  for (int xs = xlo; xs <= xhi; xs++) {
+    bool rx = false;
+    const int xw = MIR ? mirror_wrap(xs, g.nx, (g.mirror & 1) != 0, rx) : wrapi(xs, g.nx);
+    // the slice's Lagrange weight in each target's four x slices (0 outside)
+    float wx[3];
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+      const int m = xs - lx[t];
+      wx[t] = m == 0 ? Fx[t][0] : m == 1 ? Fx[t][1] : m == 2 ? Fx[t][2] : m == 3 ? Fx[t][3] : 0.f;
+    }
+#pragma unroll 1
+    for (int yr = ylo; yr <= yhi; yr++) {
+      bool ryf = false;
+      const int yw = MIR ? mirror_wrap(yr, g.ny, (g.mirror & 2) != 0, ryf) : wrapi(yr, g.ny);
+      float wy[3];
+#pragma unroll
+      for (int e = 0; e < 3; e++) {
+        const int a = yr - ly[e];
+        wy[e] = a == 0 ? Fy[e][0] : a == 1 ? Fy[e][1] : a == 2 ? Fy[e][2] : a == 3 ? Fy[e][3] : 0.f;
+      }
+#pragma unroll 1
+      for (int b = 0; b < 4; b++) {
+        bool rzf = false;
+        const int zw = MIR ? mirror_wrap(lz + b, g.nz, (g.mirror & 4) != 0, rzf) : wrapi(lz + b, g.nz);
+        const float fzb = b == 0 ? Fz[0] : b == 1 ? Fz[1] : b == 2 ? Fz[2] : Fz[3];
+        const uint32_t boff = (uint32_t)cell_index(g, xw, yw, zw) * 4u;
+        float f[27];
+        load27(src.f, boff, f);
+        Gauge4 n = ldg4(src, boff);
+        // a node beyond a mirrored face is the mirror image of the cell inside
+        // (Cell::mirrX/Y/Z, structs.cuh:552-578): its axis velocity negated, and its populations
+        // exchanged with their mirror images, which is the same as reversing that axis's basis
+        // values (node_xfer_mirrored)
+        if (MIR) {
+          n.ux = rx ? -n.ux : n.ux;
+          n.uy = ryf ? -n.uy : n.uy;
+          n.uz = rzf ? -n.uz : n.uz;
+        }
+        auto flip = [](bool m, float L[3]) {
+          const float a = L[0];
+          L[0] = m ? L[2] : L[0];
+          L[2] = m ? a : L[2];
+        };
+        float Lz[3];
+        basis3u(((u1z - n.uz) + u1zl) * sqTb, n.sq, czb, Tzb, Lz);
+        if (MIR) flip(rzf, Lz);
+        float szz[3][3];
+#pragma unroll
+        for (int ja = 0; ja < 3; ja++)
+#pragma unroll
+          for (int jb = 0; jb < 3; jb++) {
+            float sz = 0.f;
+#pragma unroll
+            for (int jc = 0; jc < 3; jc++) sz = __builtin_fmaf(Lz[jc], f[kIdx27.v[ja][jb][jc]], sz);
+            szz[ja][jb] = sz;
+          }
+        const float Dy = ((u1y - n.uy) + u1yl) * sqTb;
+        const float Dx = ((u1x - n.ux) + u1xl) * sqTb;
+        float A0 = Dx + n.sq, A1 = Dx, A2 = Dx - n.sq;
+        if (MIR) {
+          const float a = A0;
+          A0 = rx ? A2 : A0;
+          A2 = rx ? a : A2;
+        }
+        // x moments of the z sums, common to the three groups (as in interp_pass_weno5c)
+        const float B0 = A0 * A0, B1 = A1 * A1, B2 = A2 * A2;
+        float xr[3], xq[3], xp[3];
+#pragma unroll
+        for (int jb = 0; jb < 3; jb++) {
+          xr[jb] = (szz[0][jb] + szz[1][jb]) + szz[2][jb];
+          xq[jb] = __builtin_fmaf(A2, szz[2][jb], __builtin_fmaf(A1, szz[1][jb], A0 * szz[0][jb]));
+          xp[jb] = __builtin_fmaf(B2, szz[2][jb], __builtin_fmaf(B1, szz[1][jb], B0 * szz[0][jb]));
+        }
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+          if (wy[e] == 0.f) continue;
+          const int ey = e - 1;
+          float Ly[3];
+          basis3u(Dy, n.sq, ey ? (float)ey * sq1 : 0.0f, ey ? 0.0f : T1, Ly);
+          if (MIR) flip(ryf, Ly);
+          const float R = __builtin_fmaf(Ly[2], xr[2], __builtin_fmaf(Ly[1], xr[1], Ly[0] * xr[0]));
+          const float Q = __builtin_fmaf(Ly[2], xq[2], __builtin_fmaf(Ly[1], xq[1], Ly[0] * xq[0]));
+          const float Pp = __builtin_fmaf(Ly[2], xp[2], __builtin_fmaf(Ly[1], xp[1], Ly[0] * xp[0]));
+          const float Tt[3] = {__builtin_fmaf(sq1, Q, Pp), __builtin_fmaf(-T1, R, Pp), __builtin_fmaf(-sq1, Q, Pp)};
+          const float wyz = wy[e] * fzb;
+#pragma unroll
+          for (int t = 0; t < 3; t++) acc[e][t] = __builtin_fmaf(wx[t] * wyz, Tt[t], acc[e][t]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 3; e++) {
+    const float wyz = ((e != 1) ? h : mm) * wz;
+#pragma unroll
+    for (int t = 0; t < 3; t++)
+      fout[split27_fidx(kIdx27.v[t][e][ez + 1], cell, P)] = acc[e][t] * ((t != 1 ? h : mm) * wyz);
+  }
+}
+
 // Iteration state of one cell, carried from the first pass to the second.
 struct SplitIter {
   double gc[4];      // current gauge (u, T)
@@ -782,12 +964,15 @@ __device__ __forceinline__ bool split27_once(const NodeSrcF& src, float* __restr
       kahan_add(bzs, bzc, (float)ez * fn);
       cs += (double)fn * (double)(ex * ex + ey * ey + ez * ez);
     };
-    if constexpr (INTERP == 2) {
-      // WENO5C: the nine targets of each ez share their node loads and their z contraction
-      // (interp_pass_weno5c stores the values to fout); the moments are then summed in the
-      // order (ey, ez, ex) of the per-group form
+    if constexpr (INTERP == 1 || INTERP == 2 || INTERP == 3) {
+      // WENO5C and Lagrange-4: the nine targets of each ez share their node loads and their z
+      // contraction (the pass stores the values to fout); the moments are then summed in the
+      // order (ey, ez, ex)
 #pragma unroll 1
-      for (int pz = 0; pz < 3; pz++) interp_pass_weno5c(src, x, y, z, s, gc, pz - 1, sqTb, fout, cell, P);
+      for (int pz = 0; pz < 3; pz++) {
+        if constexpr (INTERP == 2) interp_pass_weno5c(src, x, y, z, s, gc, pz - 1, sqTb, fout, cell, P);
+        else interp_pass_lagr4<INTERP == 3>(src, x, y, z, s, gc, pz - 1, sqTb, fout, cell, P);
+      }
       for (int gi = 0; gi < 9; gi++) {
         const int ey = gi / 3 - 1, ez = gi % 3 - 1;
 #pragma unroll
@@ -903,20 +1088,22 @@ __device__ __forceinline__ void split27_finish(float* __restrict__ fout, float* 
   niter_out[cell] = st.niter;
 }
 
-// A predicted start is taken only in cells whose previous step took 2 .. kLearnedPrevIters
-// iterations.  Where the gauge map contracts slowly (cells that needed more, on shocks and
-// contact layers) it can have a second fixed point near the first, and a start from elsewhere
-// may settle there: on the 384^3 shock-bubble, ungated predicted starts left 13-189 cells per
-// step 1e-4 to 2e-3 (relative, u and T) away from the FP64 step taken from the same state, all
-// in cells whose FP64 step needed 9-10 iterations; gated at 2, 3, 4 or 6 previous iterations the
-// one-step error stayed at the plain start's 2e-7 on every cell.  A cell whose previous step took
-// one iteration starts from a gauge that is already a fixed point to within the tolerance, and
-// one iteration is the least any start can take.
-constexpr int kLearnedPrevIters = 4;
+// A predicted start is taken only in cells whose previous step took kLearnedPrevMin ..
+// kLearnedPrevMax iterations (step_split.h).  Where the gauge map contracts slowly (cells that
+// needed more, on shocks and contact layers) it can have a second fixed point near the first,
+// and a start from elsewhere may settle there: on the 384^3 shock-bubble, one step from the same
+// FP64 state, ungated predicted starts left 13-189 cells per step 1e-4 to 2e-3 (relative, u and
+// T) away from the FP64 step, all in cells whose FP64 step needed 9-10 iterations; with an upper
+// bound of 8 previous iterations 76-78 cells were still up to 1.7e-3 off; with 4 or 6 every cell
+// stayed within the plain start's 2e-7.  On the 64^3 shock-bubble (sharper fronts per cell) an
+// upper bound of 6 left cells 5.8e-5 off at step 12, and 4 kept every cell within 3e-7 through
+// step 12.  Below: a cell that took one or two iterations gains
+// nothing (a prediction lands within the 1e-7 tolerance of the fixed point only by chance, so
+// such a cell takes as many iterations from it; windows from 1, 2 or 3 gave the same counts).
 
 // Starting state of a cell: its previous gauge, or the predicted one where a prediction is
 // given, finite, not all zero, has T > 0, lies within the trust radius and the cell's previous
-// step took 2 .. kLearnedPrevIters iterations (niter_prev: the counts of the previous step,
+// step took kLearnedPrevMin .. kLearnedPrevMax iterations (niter_prev: the counts of the previous step,
 // which this step overwrites cell by cell after reading).
 __device__ __forceinline__ void split27_start(const float* __restrict__ min, const double* __restrict__ Tin,
                                               const float* __restrict__ guess, const int* __restrict__ niter_prev,
@@ -939,7 +1126,7 @@ __device__ __forceinline__ void split27_start(const float* __restrict__ min, con
     const float pmax = fmaxf(fmaxf(fabsf(p0), fabsf(p1)), fmaxf(fabsf(p2), fabsf(p3)));
     // (a prediction of no change at all is the previous-gauge start, with its stopping rules)
     if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0 && pmax <= a.learned_trust &&
-        pmax > 0.0f && niter_prev[cell] >= 2 && niter_prev[cell] <= kLearnedPrevIters) {
+        pmax > 0.0f && niter_prev[cell] >= kLearnedPrevMin && niter_prev[cell] <= kLearnedPrevMax) {
       for (int k = 0; k < 4; k++) st.gc[k] = q[k];
       st.learned = true;
     }
@@ -1176,7 +1363,7 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
     const float pmax = fmaxf(fmaxf(fabsf(guess[cell]), fabsf(guess[P + cell])),
                              fmaxf(fabsf(guess[2 * P + cell]), fabsf(guess[3 * P + cell])));
     if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0 && pmax <= a.learned_trust &&
-        pmax > 0.0f && niter_out[cell] >= 2 && niter_out[cell] <= kLearnedPrevIters) {
+        pmax > 0.0f && niter_out[cell] >= kLearnedPrevMin && niter_out[cell] <= kLearnedPrevMax) {
       for (int d = 0; d < 4; d++) gc[d] = q[d];
       learned = true;
     }
@@ -1349,12 +1536,15 @@ hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const
     hipLaunchKernelGGL((k_step_split27_rest<I>), dim3(blocks2), dim3(64), 0, st, in.f, in.m, in.T, g4, out.f,     \
                        out.m, out.T, niter, pend, a);                                                             \
   } while (0)
-  if (interp == 2 && a.refill_chunk > 0 && a.pass1_iters >= a.max_iter + kLearnedBudget) {
-    // (measured alternatives at 384^3: one x slice per trip 372 ms, held to 3 waves per SIMD
-    // with spills 360-400 ms, against 317 ms for two slices at 2 waves per SIMD)
+  if (interp != 0 && a.refill_chunk > 0 && a.pass1_iters >= a.max_iter + kLearnedBudget) {
     const int64_t waves = ((int64_t)blocks + a.refill_chunk - 1) / a.refill_chunk;
-    hipLaunchKernelGGL(k_step_split27_refill<2>, dim3((unsigned)waves), dim3(64), 0, st, in.f, in.m, in.T, g4, out.f,
-                       out.m, out.T, niter, guess, a, a.refill_chunk);
+#define POND_LAUNCH_R(I)                                                                                          \
+  hipLaunchKernelGGL(k_step_split27_refill<I>, dim3((unsigned)waves), dim3(64), 0, st, in.f, in.m, in.T, g4, out.f, \
+                     out.m, out.T, niter, guess, a, a.refill_chunk)
+    if (interp == 2) POND_LAUNCH_R(2);
+    else if (a.g.mirror) POND_LAUNCH_R(3);
+    else POND_LAUNCH_R(1);
+#undef POND_LAUNCH_R
     return hipGetLastError();
   }
   if (interp == 1 && a.g.mirror) POND_LAUNCH_S(k_step_split27<3>, 3);
