@@ -53,8 +53,13 @@ typedef enum { POND_D1Q5 = 0, POND_D3Q27 = 1, POND_D3Q125 = 2 } pond_lattice;
    POND_LAGR4  propagate<4>, the reference's active interpolator (steps.cu:451, 986-1036).
    POND_WENO5C the corrected WENO5 (no reference counterpart): y/z nodes floor(y+0.5)-1..+1,
                smoothness row through the departure point.  Identical to POND_WENO5 on a line
-               (Ny = Nz = 1). */
-typedef enum { POND_WENO5 = 0, POND_LAGR4 = 1, POND_WENO5C = 2 } pond_interp;
+               (Ny = Nz = 1).
+   POND_WENO5G the generic propagateWENO<5,M> (steps.cu:457, 610-654) with propagate3
+               (909-923): the nodes, smoothness samples and interpolant of POND_WENO5 with the
+               sub-stencils numbered from the right (lshift = 1) and the indicators and weights
+               in another operation order, so POND_FP64 results differ from POND_WENO5 in the
+               last bits.  POND_SPLIT runs it as POND_WENO5. */
+typedef enum { POND_WENO5 = 0, POND_LAGR4 = 1, POND_WENO5C = 2, POND_WENO5G = 3 } pond_interp;
 
 /* POND_FP64: plain path, reference operation order, FP64 everywhere.
    POND_SPLIT: FP32 populations/streaming/interpolation/collide, compensated moment sums,
@@ -104,15 +109,27 @@ typedef struct {
      applies to POND_LAGR4 (D1Q5, D3Q27) and to D3Q125; WENO5 interpolation with a mirror is
      rejected (POND_ERR_INVALID). */
   int mirror;
+  /* 1: the 1D cell activation of amr.cu after every step, as calcStep would run it with its
+     launches enabled (update.cu:102-105): a gap between active cells across which the new states
+     differ (|du_x|^2/min T > 0.01 or |rho ratio - 1| > 0.01) and that is an even number of cells
+     wide gets the cell half way switched on with the average record (AMR_add, amr.cu:1-27);
+     then one pass along the line in increasing x switches off each active cell whose previous
+     and next active cells are within 3 cells and match it (AMR_remove, amr.cu:28-55).  A
+     switched-off cell is not stepped (its record is left as it was, LBMpull, steps.cu:359) but
+     stays an interpolation node.  D1Q5 lines (ny = nz = 1) of up to 32768 cells, POND_FP64
+     only (POND_ERR_INVALID otherwise).  Default 0. */
+  int amr;
 } pond_params;
 
 typedef struct {
   double mass;       /* sum rho                       (update.cu:65) */
   double momentum_x; /* sum rho*ux                    (update.cu:66) */
   double energy;     /* sum rho*(|u|^2 + Dim*T)       (update.cu:67) */
-  double k;          /* kinetic energy of diagn.cu:18 */
+  double k;          /* kinetic energy of diagn.cu:18 (every cell) */
   int32_t niter_max; /* max gauge iterations, last step (steps.cu:513) */
-  double niter_mean; /* mean gauge iterations per cell, last step */
+  double niter_mean; /* mean gauge iterations per cell, last step
+                        (mass, momentum, energy, the iteration figures and max_departure cover
+                        the active cells, pond_get_active) */
   int64_t step;      /* steps taken (LBMParams::iStep) */
   double max_departure; /* largest population displacement per step along an axis, in cells:
                            max over cells and axes of |u_d| + sqrt(T/TLat)*max|e_d| (the
@@ -160,6 +177,13 @@ pond_status pond_set_populations(pond_ctx* ctx, const double* f, const double* r
    counterpart in the reference (one global state); it lets a run switch precision, and lets one
    step of two paths start from the same state. */
 pond_status pond_copy_state(pond_ctx* dst, pond_ctx* src);
+
+/* Cinfo::set of every cell (structs.cuh:92): 1 = stepped, 0 = switched off (kept as it is, still
+   an interpolation node).  Every cell starts switched on (fill, init.cu:106); pond_params.amr
+   switches cells on and off after each step.  pond_set_active applies to D1Q5 POND_FP64
+   contexts (the cells of other contexts are always on; POND_ERR_INVALID). */
+pond_status pond_get_active(pond_ctx* ctx, int32_t* set);
+pond_status pond_set_active(pond_ctx* ctx, const int32_t* set);
 
 /* nsteps PonD steps (calcStep without I/O).  Asynchronous on the ctx stream; returns after
    the launches are queued.  pond_sync / any get call waits. */

@@ -89,7 +89,7 @@ __global__ void k_copy_state(StateView in, StateOut out, int64_t n, int64_t plan
 
 // block-level deterministic partial sums: mass, momentum_x, energy, k, niter sum; niter max
 struct RedOut {
-  double s[5];
+  double s[6];  // mass, momentum_x, energy, k, sum of iteration counts, active cells
   double vmax;  // largest |u_d| + sqrt(|T|/TLat)*emax
   int imax;
 };
@@ -135,28 +135,32 @@ __device__ __forceinline__ double departure(double ux, double uy, double uz, dou
   return fmax(fmax(fabs(ux), fabs(uy)), fabs(uz)) + sqrt(fabs(T) * inv_tlat) * emax;
 }
 
-// calcTotMoments (update.cu:52-69) + diagn (diagn.cu:3-22) + NiterMax, for the FP64 state
+// calcTotMoments (update.cu:52-69, active cells) + diagn (diagn.cu:3-22, every cell) + NiterMax
+// (steps.cu:513, stepped cells), for the FP64 state; set: Cinfo::set or null (every cell on)
 __global__ __launch_bounds__(256) void k_reduce_f64(const double* __restrict__ m, const int* __restrict__ niter,
-                                                    int64_t n, int64_t plane, double dim, double cx, double cy,
-                                                    double cz, double inv_tlat, double emax, RedOut* __restrict__ out) {
+                                                    const int* __restrict__ set, int64_t n, int64_t plane, double dim,
+                                                    double cx, double cy, double cz, double inv_tlat, double emax,
+                                                    RedOut* __restrict__ out) {
   __shared__ double shd[4];
   __shared__ int shi[4];
-  double s[5] = {0, 0, 0, 0, 0};
+  double s[6] = {0, 0, 0, 0, 0, 0};
   double vmax = 0;
   int imax = 0;
   const double d8pi3 = 1.0 / (8 * M_PI * M_PI * M_PI);
   for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
     const double rho = m[c], ux = m[plane + c], uy = m[2 * plane + c], uz = m[3 * plane + c], T = m[4 * plane + c];
+    s[3] += 0.5 * d8pi3 * (ux * ux + uy * uy + uz * uz) * cx * cy * cz;
+    if (set && !set[c]) continue;
     s[0] += rho * 1.0;
     s[1] += rho * ux * 1.0;
     s[2] += rho * (ux * ux + uy * uy + uz * uz + dim * T) * 1.0;
-    s[3] += 0.5 * d8pi3 * (ux * ux + uy * uy + uz * uz) * cx * cy * cz;
     s[4] += (double)niter[c];
+    s[5] += 1.0;
     imax = max(imax, niter[c]);
     vmax = fmax(vmax, departure(ux, uy, uz, T, inv_tlat, emax));
   }
   RedOut r;
-  for (int q = 0; q < 5; q++) r.s[q] = block_sum(s[q], shd);
+  for (int q = 0; q < 6; q++) r.s[q] = block_sum(s[q], shd);
   r.imax = block_max(imax, shi);
   r.vmax = block_maxd(vmax, shd);
   if (threadIdx.x == 0) out[blockIdx.x] = r;
@@ -169,12 +173,13 @@ __global__ __launch_bounds__(256) void k_reduce_split(const float* __restrict__ 
                                                       double emax, RedOut* __restrict__ out) {
   __shared__ double shd[4];
   __shared__ int shi[4];
-  double s[5] = {0, 0, 0, 0, 0};
+  double s[6] = {0, 0, 0, 0, 0, 0};
   double vmax = 0;
   int imax = 0;
   const double d8pi3 = 1.0 / (8 * M_PI * M_PI * M_PI);
   for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
     const double rho = m[c], ux = m[plane + c], uy = m[2 * plane + c], uz = m[3 * plane + c], T = Tp[c];
+    s[5] += 1.0;
     s[0] += rho;
     s[1] += rho * ux;
     s[2] += rho * (ux * ux + uy * uy + uz * uz + dim * T);
@@ -184,7 +189,7 @@ __global__ __launch_bounds__(256) void k_reduce_split(const float* __restrict__ 
     vmax = fmax(vmax, departure(ux, uy, uz, T, inv_tlat, emax));
   }
   RedOut r;
-  for (int q = 0; q < 5; q++) r.s[q] = block_sum(s[q], shd);
+  for (int q = 0; q < 6; q++) r.s[q] = block_sum(s[q], shd);
   r.imax = block_max(imax, shi);
   r.vmax = block_maxd(vmax, shd);
   if (threadIdx.x == 0) out[blockIdx.x] = r;
@@ -194,7 +199,7 @@ __global__ void k_reduce_final(const RedOut* __restrict__ part, int nb, RedOut* 
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   RedOut r{};
   for (int b = 0; b < nb; b++) {
-    for (int q = 0; q < 5; q++) r.s[q] += part[b].s[q];
+    for (int q = 0; q < 6; q++) r.s[q] += part[b].s[q];
     r.imax = max(r.imax, part[b].imax);
     r.vmax = fmax(r.vmax, part[b].vmax);
   }
@@ -305,7 +310,8 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
   // node data are addressed with 32-bit byte offsets into 8-byte planes
   if (p->nx * p->ny * p->nz > ((int64_t)1 << 29) - 64)
     return bail(POND_ERR_INVALID, "pond_create: at most 2^29 - 64 cells per context");
-  if (p->interp != POND_WENO5 && p->interp != POND_LAGR4 && p->interp != POND_WENO5C) return bail(POND_ERR_INVALID, "pond_create: unknown interp");
+  if (p->interp != POND_WENO5 && p->interp != POND_LAGR4 && p->interp != POND_WENO5C && p->interp != POND_WENO5G)
+    return bail(POND_ERR_INVALID, "pond_create: unknown interp");
   if (p->precision != POND_FP64 && p->precision != POND_SPLIT)
     return bail(POND_ERR_INVALID, "pond_create: unknown precision");
   if (p->precision == POND_SPLIT && p->lattice == POND_D1Q5)
@@ -329,6 +335,11 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
   if (p->mirror && p->lattice != POND_D3Q125 && p->interp != POND_LAGR4)
     return bail(POND_ERR_INVALID,
                 "pond_create: mirror boundaries apply to POND_LAGR4 and D3Q125 (the reference's WENO5 has no mirror path)");
+  if (p->amr != 0 && p->amr != 1) return bail(POND_ERR_INVALID, "pond_create: amr must be 0 or 1");
+  if (p->amr && (p->lattice != POND_D1Q5 || p->precision != POND_FP64 || p->ny != 1 || p->nz != 1 ||
+                 p->nx > amr_max_line()))
+    return bail(POND_ERR_INVALID, "pond_create: amr applies to POND_FP64 D1Q5 lines (ny = nz = 1) of up to " +
+                                      std::to_string(amr_max_line()) + " cells");
   // row patches where the grid divides into them (ignored elsewhere)
   const int rp = p->tile_patch;
   g.rp = (rp > 0 && g.nx % 64 == 0 && g.ny % rp == 0 && g.nz % rp == 0) ? rp : 0;
@@ -440,6 +451,16 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
   ctx->bytes += (int64_t)g.plane * sizeof(int) + ctx->red_len * (int64_t)sizeof(RedOut);
   if (hipMemsetAsync(ctx->niter, 0, (size_t)g.plane * sizeof(int), ctx->stream) != hipSuccess)
     return bail(POND_ERR_HIP, "pond_create: hipMemset failed");
+  if (p->lattice == POND_D1Q5 && p->precision == POND_FP64) {
+    // Cinfo::set, every cell on (fill, init.cu:106)
+    if (hipMalloc(&ctx->cell_set, (size_t)g.plane * sizeof(int)) != hipSuccess ||
+        hipMalloc(&ctx->cell_setnew, (size_t)g.plane * sizeof(int)) != hipSuccess)
+      return bail(POND_ERR_NOMEM, "pond_create: hipMalloc of the cell flags failed");
+    ctx->bytes += (int64_t)g.plane * 2 * sizeof(int);
+    if (hipMemsetD32Async((hipDeviceptr_t)ctx->cell_set, 1, (size_t)g.plane, ctx->stream) != hipSuccess)
+      return bail(POND_ERR_HIP, "pond_create: hipMemset failed");
+    ctx->args.cell_set = ctx->cell_set;
+  }
   if (ensure_events(ctx) != POND_OK) return bail(POND_ERR_HIP, "pond_create: " + ctx->err);
   *out = ctx;
   return POND_OK;
@@ -459,6 +480,8 @@ void pond_destroy(pond_ctx* ctx) {
   for (auto& e : ctx->mlp_ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->niter) (void)hipFree(ctx->niter);
+  if (ctx->cell_set) (void)hipFree(ctx->cell_set);
+  if (ctx->cell_setnew) (void)hipFree(ctx->cell_setnew);
   if (ctx->sq) (void)hipFree(ctx->sq);
   if (ctx->nodeP) (void)hipFree(ctx->nodeP);
   if (ctx->nodeok) (void)hipFree(ctx->nodeok);
@@ -574,6 +597,31 @@ pond_status pond_set_populations(pond_ctx* ctx, const double* f, const double* r
   return POND_OK;
 }
 
+pond_status pond_get_active(pond_ctx* ctx, int32_t* set) {
+  if (!ctx || !set) return POND_ERR_INVALID;
+  pond_status s = pond_sync(ctx);
+  if (s != POND_OK) return s;
+  const int64_t n = ctx->g.n;
+  if (!ctx->cell_set) {
+    for (int64_t c = 0; c < n; c++) set[c] = 1;
+    return POND_OK;
+  }
+  POND_HIP(ctx, hipMemcpy(set, ctx->cell_set, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return POND_OK;
+}
+
+pond_status pond_set_active(pond_ctx* ctx, const int32_t* set) {
+  if (!ctx || !set) return POND_ERR_INVALID;
+  if (!ctx->cell_set)
+    return fail(ctx, POND_ERR_INVALID, "pond_set_active: cells can be switched off in POND_FP64 D1Q5 contexts only");
+  const int64_t n = ctx->g.n;
+  std::vector<int32_t> h((size_t)n);
+  for (int64_t c = 0; c < n; c++) h[(size_t)c] = set[c] != 0;
+  POND_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  POND_HIP(ctx, hipMemcpy(ctx->cell_set, h.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
+  return POND_OK;
+}
+
 pond_status pond_copy_state(pond_ctx* dst, pond_ctx* src) {
   if (!dst || !src) return POND_ERR_INVALID;
   if (!src->has_state) return fail(dst, POND_ERR_STATE, "pond_copy_state: the source has no state");
@@ -598,6 +646,9 @@ pond_status pond_copy_state(pond_ctx* dst, pond_ctx* src) {
   }
   POND_HIP(dst, hipMemcpyAsync(dst->niter, src->niter, (size_t)g.plane * sizeof(int), hipMemcpyDeviceToDevice,
                                dst->stream));
+  if (dst->cell_set && src->cell_set)
+    POND_HIP(dst, hipMemcpyAsync(dst->cell_set, src->cell_set, (size_t)g.plane * sizeof(int),
+                                 hipMemcpyDeviceToDevice, dst->stream));
   POND_HIP(dst, hipStreamSynchronize(dst->stream));
   dst->cur = 0;
   dst->has_state = true;
@@ -661,7 +712,10 @@ pond_status pond_step(pond_ctx* ctx, int nsteps) {
     if (ctx->split) {
       const SplitPtrs pin{ctx->sp[in].f, ctx->sp[in].m, ctx->sp[in].T};
       const SplitPtrs pout{ctx->sp[outb].f, ctx->sp[outb].m, ctx->sp[outb].T};
-      he = launch_step_split(ctx->p.lattice, ctx->p.interp, pin, pout, ctx->sq, ctx->niter,
+      // the generic WENO5 is the as-written interpolant in another FP64 operation order; the
+      // split path takes the as-written kernel for it
+      he = launch_step_split(ctx->p.lattice, ctx->p.interp == POND_WENO5G ? POND_WENO5 : ctx->p.interp, pin, pout,
+                             ctx->sq, ctx->niter,
                              learned ? ctx->guess : nullptr, pending_view(ctx), ctx->sargs, ctx->stream);
     } else {
       he = launch_step_f64(ctx->p.lattice, ctx->p.interp, ctx->f64[in].f, ctx->f64[in].m, ctx->nodeP, ctx->nodeok,
@@ -670,6 +724,10 @@ pond_status pond_step(pond_ctx* ctx, int nsteps) {
     if (he != hipSuccess) return fail(ctx, POND_ERR_HIP, std::string("step kernel launch: ") + hipGetErrorString(he));
     POND_HIP(ctx, hipEventRecord(e.b, ctx->stream));
     ctx->ev_pending.push_back(e);
+    if (ctx->p.amr) {
+      he = launch_amr(ctx->f64[outb], ctx->f64[in], ctx->cell_set, ctx->cell_setnew, ctx->g, ctx->Qn, ctx->stream);
+      if (he != hipSuccess) return fail(ctx, POND_ERR_HIP, std::string("amr kernels: ") + hipGetErrorString(he));
+    }
     ctx->cur = outb;  // swapLS (structs.cuh:148)
     ctx->step++;
   }
@@ -746,9 +804,9 @@ pond_status pond_diagnostics(pond_ctx* ctx, pond_diag* out) {
                        ctx->niter, g.n, g.plane, ctx->L.Dim, 2 * M_PI / g.nx, 2 * M_PI / g.ny, 2 * M_PI / g.nz,
                        1.0 / ctx->L.TLat, emax, part);
   else
-    hipLaunchKernelGGL(k_reduce_f64, dim3(nb), dim3(256), 0, ctx->stream, ctx->f64[ctx->cur].m, ctx->niter, g.n,
-                       g.plane, ctx->L.Dim, 2 * M_PI / g.nx, 2 * M_PI / g.ny, 2 * M_PI / g.nz, 1.0 / ctx->L.TLat, emax,
-                       part);
+    hipLaunchKernelGGL(k_reduce_f64, dim3(nb), dim3(256), 0, ctx->stream, ctx->f64[ctx->cur].m, ctx->niter,
+                       (const int*)ctx->cell_set, g.n, g.plane, ctx->L.Dim, 2 * M_PI / g.nx, 2 * M_PI / g.ny,
+                       2 * M_PI / g.nz, 1.0 / ctx->L.TLat, emax, part);
   POND_HIP(ctx, hipGetLastError());
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, ctx->stream, part, nb, part + kRedBlocks);
   POND_HIP(ctx, hipGetLastError());
@@ -760,7 +818,7 @@ pond_status pond_diagnostics(pond_ctx* ctx, pond_diag* out) {
   out->momentum_x = r.s[1];
   out->energy = r.s[2];
   out->k = r.s[3];
-  out->niter_mean = ctx->step > 0 ? r.s[4] / (double)g.n : 0.0;
+  out->niter_mean = ctx->step > 0 && r.s[5] > 0 ? r.s[4] / r.s[5] : 0.0;
   out->niter_max = ctx->step > 0 ? r.imax : 0;
   out->step = ctx->step;
   out->max_departure = r.vmax;
@@ -813,6 +871,9 @@ struct CkptHeader {
   int32_t version, lattice, precision, interp;
   int64_t nx, ny, nz, plane, step;
   int64_t fbytes, mbytes, tbytes, nbytes;  // section sizes: populations, moments, T, niter
+  // contexts with switchable cells (D1Q5 FP64): the cell flags, and the other buffer's
+  // populations and moments, whose switched-off cells hold their own (older) records
+  int64_t sbytes;
 };
 
 struct CkptSections {
@@ -848,16 +909,19 @@ pond_status pond_checkpoint_write(pond_ctx* ctx, const char* path) {
   const CkptSections c = ckpt_sections(ctx, ctx->cur);
   CkptHeader h{};
   std::memcpy(h.magic, "PONDCKPT", 8);
-  h.version = 1; h.lattice = ctx->p.lattice; h.precision = ctx->p.precision; h.interp = ctx->p.interp;
+  h.version = 2; h.lattice = ctx->p.lattice; h.precision = ctx->p.precision; h.interp = ctx->p.interp;
   h.nx = g.nx; h.ny = g.ny; h.nz = g.nz; h.plane = g.plane; h.step = ctx->step;
   h.fbytes = c.fbytes; h.mbytes = c.mbytes; h.tbytes = c.tbytes; h.nbytes = c.nbytes;
+  h.sbytes = ctx->cell_set ? g.plane * (int64_t)sizeof(int) : 0;
+  const CkptSections o = ckpt_sections(ctx, 1 - ctx->cur);
   FILE* fp = std::fopen(path, "wb");
   if (!fp) return fail(ctx, POND_ERR_IO, std::string("pond_checkpoint_write: cannot open ") + path);
   bool ok = std::fwrite(&h, sizeof h, 1, fp) == 1;
-  const void* src[4] = {c.f, c.m, c.T, ctx->niter};
-  const int64_t len[4] = {c.fbytes, c.mbytes, c.tbytes, c.nbytes};
+  const void* src[7] = {c.f, c.m, c.T, ctx->niter, o.f, o.m, ctx->cell_set};
+  const int64_t len[7] = {c.fbytes, c.mbytes, c.tbytes, c.nbytes, h.sbytes ? c.fbytes : 0, h.sbytes ? c.mbytes : 0,
+                          h.sbytes};
   std::vector<char> buf;
-  for (int k = 0; k < 4 && ok; k++) {
+  for (int k = 0; k < 7 && ok; k++) {
     if (len[k] == 0) continue;
     buf.resize((size_t)len[k]);
     if (hipMemcpy(buf.data(), src[k], (size_t)len[k], hipMemcpyDeviceToHost) != hipSuccess) {
@@ -880,32 +944,37 @@ pond_status pond_checkpoint_read(pond_ctx* ctx, const char* path) {
   CkptHeader h{};
   const CkptSections c = ckpt_sections(ctx, 0);
   bool ok = std::fread(&h, sizeof h, 1, fp) == 1;
-  if (!ok || std::memcmp(h.magic, "PONDCKPT", 8) != 0 || h.version != 1) {
+  if (!ok || std::memcmp(h.magic, "PONDCKPT", 8) != 0 || h.version != 2) {
     std::fclose(fp);
     return fail(ctx, POND_ERR_IO, std::string("pond_checkpoint_read: not a checkpoint: ") + path);
   }
   if (h.lattice != ctx->p.lattice || h.precision != ctx->p.precision || h.nx != g.nx || h.ny != g.ny ||
       h.nz != g.nz || h.plane != g.plane || h.fbytes != c.fbytes || h.mbytes != c.mbytes || h.tbytes != c.tbytes ||
-      h.nbytes != c.nbytes) {
+      h.nbytes != c.nbytes || h.sbytes != (ctx->cell_set ? g.plane * (int64_t)sizeof(int) : 0)) {
     std::fclose(fp);
     return fail(ctx, POND_ERR_INVALID, "pond_checkpoint_read: lattice, precision or grid differ from the context");
   }
-  const int64_t len[4] = {c.fbytes, c.mbytes, c.tbytes, c.nbytes};
-  std::vector<char> sec[4];
-  for (int k = 0; k < 4 && ok; k++) {
+  const int64_t len[7] = {c.fbytes, c.mbytes, c.tbytes, c.nbytes, h.sbytes ? c.fbytes : 0, h.sbytes ? c.mbytes : 0,
+                          h.sbytes};
+  std::vector<char> sec[7];
+  for (int k = 0; k < 7 && ok; k++) {
     sec[k].resize((size_t)len[k]);
     if (len[k]) ok = std::fread(sec[k].data(), 1, (size_t)len[k], fp) == (size_t)len[k];
   }
   std::fclose(fp);
   if (!ok) return fail(ctx, POND_ERR_IO, std::string("pond_checkpoint_read: truncated file: ") + path);
-  // both buffers hold the state, as after pond_set_state
+  // buffer 0 holds the state; buffer 1 the other buffer's records where the file has them, else
+  // the state again, as after pond_set_state
   for (int b = 0; b < 2; b++) {
     const CkptSections d = ckpt_sections(ctx, b);
     void* dst[3] = {d.f, d.m, d.T};
-    for (int k = 0; k < 3; k++)
-      if (len[k]) POND_HIP(ctx, hipMemcpy(dst[k], sec[k].data(), (size_t)len[k], hipMemcpyHostToDevice));
+    for (int k = 0; k < 3; k++) {
+      const int from = (b == 1 && h.sbytes && k < 2) ? 4 + k : k;
+      if (len[k]) POND_HIP(ctx, hipMemcpy(dst[k], sec[from].data(), (size_t)len[k], hipMemcpyHostToDevice));
+    }
   }
   POND_HIP(ctx, hipMemcpy(ctx->niter, sec[3].data(), (size_t)len[3], hipMemcpyHostToDevice));
+  if (h.sbytes) POND_HIP(ctx, hipMemcpy(ctx->cell_set, sec[6].data(), (size_t)len[6], hipMemcpyHostToDevice));
   ctx->cur = 0;
   ctx->step = h.step;
   ctx->has_state = true;

@@ -61,6 +61,11 @@ hipError_t launch_node_prep(int lattice, const double* f, const double* m, doubl
 hipError_t launch_step_f64(int lattice, int interp, const double* fin, const double* min, const double* Pin,
                            const unsigned char* okin, double* fout, double* mout, int* niter, const Pending& pend,
                            const StepArgs& a, hipStream_t st);
+// the cell activation passes after a D1Q5 FP64 step (amr.hip): st = the step's output buffer,
+// ld = its input buffer
+hipError_t launch_amr(const F64Buf& st, const F64Buf& ld, int* set, int* setnew, const Grid& g, int qn,
+                      hipStream_t stream);
+int amr_max_line();
 
 }  // namespace pond
 
@@ -87,6 +92,8 @@ struct pond_ctx {
   char* pend = nullptr;
   int64_t pend_cap = 0;
   int* niter = nullptr;     // per-cell gauge iterations of the last step
+  int* cell_set = nullptr;  // D1Q5 FP64: Cinfo::set per cell (1 = stepped) and its next value
+  int* cell_setnew = nullptr;
   double* red = nullptr;    // reduction partials
   int64_t red_len = 0;
   int64_t step = 0;

@@ -35,6 +35,7 @@ struct StepArgs {
   int pass1_iters;    // iterations of the first pass of the one-lane-per-cell step
   int global_loads;   // 1: global loads even where the buffer-resource loads apply
   int refill_chunk;   // D3Q27 first pass: 64-cell tiles per wave with lane refill (0: one tile per wave)
+  const int* cell_set;  // Cinfo::set per cell (0: not stepped), or null: every cell is stepped
   double w[kQmax];    // lattice weights for the collide and set_base
 };
 

@@ -409,16 +409,22 @@ __device__ __forceinline__ void lag4_factors(double s, double F[4][3]) {
   }
 }
 
-// propagateWENO<3,5> (steps.cu:774-825) with LagrEval<3> (steps.cu:924-937).
-// CENTRED=false: as written, y/z sub-stencil nodes floor(y+0.5)+0..2 and smoothness samples on
-// the row (r mod Ny, r mod Nz), r = floor(x+0.5) (steps.cu:776-783).  This extrapolates
-// downwind in y and z and is unstable in 3D (grid-scale modes grow every step).
-// CENTRED=true: the corrected variant (not in the reference): y/z nodes floor(y+0.5)-1..+1 and
-// smoothness samples on the departure point's own row, which is the middle node of every
-// slice, so they reuse that slice's transfers.
-template <int LAT, bool CENTRED, class SRC>
+// WENO5 interpolation, three forms (VAR):
+// 0: propagateWENO<3,5> (steps.cu:774-825) with LagrEval<3> (steps.cu:924-937), as written:
+//    y/z sub-stencil nodes floor(y+0.5)+0..2 and smoothness samples on the row
+//    (r mod Ny, r mod Nz), r = floor(x+0.5) (steps.cu:776-783).  This extrapolates downwind in
+//    y and z and is unstable in 3D (grid-scale modes grow every step).
+// 1: the corrected variant (not in the reference): y/z nodes floor(y+0.5)-1..+1 and smoothness
+//    samples on the departure point's own row, which is the middle node of every slice, so
+//    they reuse that slice's transfers.
+// 2: the generic propagateWENO<5,M> (steps.cu:610-654) with propagate3 (steps.cu:909-923): the
+//    nodes, samples and interpolant of form 0, with the sub-stencils numbered from the right
+//    (lshift = 1), the indicators written expanded and each weight formed in one expression;
+//    only the rounding differs from form 0.
+template <int LAT, int VAR, class SRC>
 __device__ double interp_weno5(const SRC& src, double cx, double cy, double cz, const typename Lat<LAT>::Tgt& t,
                                double sqTb) {
+  constexpr bool CENTRED = VAR == 1;
   const Grid& g = src.g;
   const int r = floor_int(cx + 0.5);
   const int ry = floor_int(cy + 0.5) - (CENTRED ? 1 : 0), rz = floor_int(cz + 0.5) - (CENTRED ? 1 : 0);
@@ -487,10 +493,27 @@ __device__ double interp_weno5(const SRC& src, double cx, double cy, double cz, 
     }
   }
   const double cdist = cx - r;
+  const double u1 = ub[0], u2 = ub[1], u3 = ub[2], u4 = ub[3], u5 = ub[4];
+  if (VAR == 2) {
+    // S0, S1, S2 of steps.cu:617-619 are this function's right, middle and left sums
+    const double d0 = (cdist * cdist + 3 * cdist + 2) / 12.;
+    const double d2 = (cdist * cdist - 3 * cdist + 2) / 12.;
+    const double d1 = 1. - d0 - d2;
+    const double beta0 = (10 * u3 * u3 - 31 * u3 * u4 + 25 * u4 * u4 + 11 * u3 * u5 - 19 * u4 * u5 + 4 * u5 * u5) / 3.0;
+    const double beta1 = (4 * u2 * u2 - 13 * u2 * u3 + 13 * u3 * u3 + 5 * u2 * u4 - 13 * u3 * u4 + 4 * u4 * u4) / 3.0;
+    const double beta2 = (4 * u1 * u1 - 19 * u1 * u2 + 25 * u2 * u2 + 11 * u1 * u3 - 31 * u2 * u3 + 10 * u3 * u3) / 3.0;
+    double w0 = d0 / ((1e-6 + beta0) * (1e-6 + beta0));
+    double w1 = d1 / ((1e-6 + beta1) * (1e-6 + beta1));
+    double w2 = d2 / ((1e-6 + beta2) * (1e-6 + beta2));
+    const double wsum = w0 + w1 + w2;
+    w0 /= wsum;
+    w1 /= wsum;
+    w2 /= wsum;
+    return w0 * S2 + w1 * S1 + w2 * S0;
+  }
   const double d0 = (cdist * cdist - 3 * cdist + 2) / 12.;
   const double d2 = (cdist * cdist + 3 * cdist + 2) / 12.;
   const double d1 = 1. - d0 - d2;
-  const double u1 = ub[0], u2 = ub[1], u3 = ub[2], u4 = ub[3], u5 = ub[4];
   const double beta0 = (10 * u3 * u3 + (11 * u1 - 31 * u2) * u3 + 25 * u2 * u2 - 19 * u1 * u2 + 4 * u1 * u1) / 3.0;
   const double beta1 = (4 * u4 * u4 + (5 * u2 - 13 * u3) * u4 + 13 * u3 * u3 - 13 * u2 * u3 + 4 * u2 * u2) / 3.0;
   const double beta2 = (4 * u5 * u5 + (11 * u3 - 19 * u4) * u5 + 25 * u4 * u4 - 31 * u3 * u4 + 10 * u3 * u3) / 3.0;
@@ -591,11 +614,12 @@ __device__ __forceinline__ bool f64_once(const SRC& src, const NodeSrc& gsrc, do
       const double vz = s * Lat<LAT>::efc(i, 2) + uT[2];
       const typename Lat<LAT>::Tgt t = Lat<LAT>::tgt(i, uT);
       const double cx = (double)x - vx, cy = (double)y - vy, cz = (double)z - vz;
-      // INTERP 3: Lagrange-4 with mirror boundaries
+      // INTERP 3: Lagrange-4 with mirror boundaries; 4: the generic WENO5
       const double fn = INTERP == 1   ? interp_lagr4<LAT, false>(src, cx, cy, cz, t, a.sqrt_tbase)
                         : INTERP == 3 ? interp_lagr4<LAT, true>(src, cx, cy, cz, t, a.sqrt_tbase)
-                        : INTERP == 2 ? interp_weno5<LAT, true>(src, cx, cy, cz, t, a.sqrt_tbase)
-                                      : interp_weno5<LAT, false>(src, cx, cy, cz, t, a.sqrt_tbase);
+                        : INTERP == 2 ? interp_weno5<LAT, 1>(src, cx, cy, cz, t, a.sqrt_tbase)
+                        : INTERP == 4 ? interp_weno5<LAT, 2>(src, cx, cy, cz, t, a.sqrt_tbase)
+                                      : interp_weno5<LAT, 0>(src, cx, cy, cz, t, a.sqrt_tbase);
       rho += fn;
       m0 += fn * vx;
       m1 += fn * vy;
@@ -680,8 +704,11 @@ __global__ __launch_bounds__(64) void k_step_f64(const NodeSrc gsrc, double* __r
   else src = gsrc;
   const Grid& g = a.g;
   int x, y, z;
-  const bool active = tile_cell(g, xcd_block(blockIdx.x, gridDim.x), threadIdx.x, x, y, z);
+  bool active = tile_cell(g, xcd_block(blockIdx.x, gridDim.x), threadIdx.x, x, y, z);
   const int64_t cell = active ? cell_index(g, x, y, z) : 0;
+  // a switched-off cell is not stepped: its output record and iteration count keep what they
+  // held (LBMpull, steps.cu:359)
+  if (active && a.cell_set && a.cell_set[cell] == 0) active = false;
   bool pending = false;
   F64Iter st;
   if (active) {
@@ -1032,30 +1059,35 @@ hipError_t launch_step_f64(int lattice, int interp, const double* fin, const dou
     hipLaunchKernelGGL((k_step_f64_rest<L, I, true>), dim3(blocks2), dim3(64), 0, st, src, fout, mout, niter,    \
                        pend, a);                                                                                 \
   } while (0)
-  // Lagrange-4 with mirror boundaries is its own instantiation (3), so the periodic kernels
-  // carry none of the mirror code
-  if (interp == 1 && a.g.mirror) interp = 3;
+  // kernel codes: 0 WENO5 as written, 1 Lagrange-4, 2 WENO5 centred, 3 Lagrange-4 with mirror
+  // boundaries (its own instantiation, so the periodic kernels carry none of the mirror code),
+  // 4 the generic WENO5.  interp is pond_interp (POND_WENO5G = 3).
+  const int code = interp == 3 ? 4 : (interp == 1 && a.g.mirror) ? 3 : interp;
   if (lattice == kD3Q27 && buf && a.refill_chunk > 0) {
-    if (interp == 0) POND_LAUNCH_R(kD3Q27, 0);
-    else if (interp == 1) POND_LAUNCH_R(kD3Q27, 1);
-    else if (interp == 3) POND_LAUNCH_R(kD3Q27, 3);
+    if (code == 0) POND_LAUNCH_R(kD3Q27, 0);
+    else if (code == 1) POND_LAUNCH_R(kD3Q27, 1);
+    else if (code == 3) POND_LAUNCH_R(kD3Q27, 3);
+    else if (code == 4) POND_LAUNCH_R(kD3Q27, 4);
     else POND_LAUNCH_R(kD3Q27, 2);
   } else if (lattice == kD3Q27) {
     if (buf) {
-      if (interp == 0) POND_LAUNCH(kD3Q27, 0, true);
-      else if (interp == 1) POND_LAUNCH(kD3Q27, 1, true);
-      else if (interp == 3) POND_LAUNCH(kD3Q27, 3, true);
+      if (code == 0) POND_LAUNCH(kD3Q27, 0, true);
+      else if (code == 1) POND_LAUNCH(kD3Q27, 1, true);
+      else if (code == 3) POND_LAUNCH(kD3Q27, 3, true);
+      else if (code == 4) POND_LAUNCH(kD3Q27, 4, true);
       else POND_LAUNCH(kD3Q27, 2, true);
     } else {
-      if (interp == 0) POND_LAUNCH(kD3Q27, 0, false);
-      else if (interp == 1) POND_LAUNCH(kD3Q27, 1, false);
-      else if (interp == 3) POND_LAUNCH(kD3Q27, 3, false);
+      if (code == 0) POND_LAUNCH(kD3Q27, 0, false);
+      else if (code == 1) POND_LAUNCH(kD3Q27, 1, false);
+      else if (code == 3) POND_LAUNCH(kD3Q27, 3, false);
+      else if (code == 4) POND_LAUNCH(kD3Q27, 4, false);
       else POND_LAUNCH(kD3Q27, 2, false);
     }
   } else {
     // on a line (Ny = Nz = 1) the centred and as-written WENO5 coincide
-    if (interp == 1) POND_LAUNCH(kD1Q5, 1, false);
-    else if (interp == 3) POND_LAUNCH(kD1Q5, 3, false);
+    if (code == 1) POND_LAUNCH(kD1Q5, 1, false);
+    else if (code == 3) POND_LAUNCH(kD1Q5, 3, false);
+    else if (code == 4) POND_LAUNCH(kD1Q5, 4, false);
     else POND_LAUNCH(kD1Q5, 0, false);
   }
 #undef POND_LAUNCH

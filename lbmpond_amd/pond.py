@@ -23,7 +23,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpond.so")
 
 LATTICES = {"D1Q5": 0, "D3Q27": 1, "D3Q125": 2}
-INTERPS = {"weno5": 0, "lagr4": 1, "weno5c": 2}
+INTERPS = {"weno5": 0, "lagr4": 1, "weno5c": 2, "weno5g": 3}
 PRECISIONS = {"fp64": 0, "split": 1}
 QN = {"D1Q5": 5, "D3Q27": 27, "D3Q125": 125}
 
@@ -60,6 +60,7 @@ class PondParams(ctypes.Structure):
         ("global_loads", ctypes.c_int),
         ("learned_trust", ctypes.c_double),
         ("mirror", ctypes.c_int),
+        ("amr", ctypes.c_int),
     ]
 
 
@@ -103,6 +104,8 @@ SIGNATURES = {
     "pond_set_state": (ctypes.c_int, [_CTX, _D, _D, _D, _D, _D]),
     "pond_set_populations": (ctypes.c_int, [_CTX, _D, _D, _D, _D, _D, _D]),
     "pond_copy_state": (ctypes.c_int, [_CTX, _CTX]),
+    "pond_get_active": (ctypes.c_int, [_CTX, _I32]),
+    "pond_set_active": (ctypes.c_int, [_CTX, _I32]),
     "pond_step": (ctypes.c_int, [_CTX, ctypes.c_int]),
     "pond_sync": (ctypes.c_int, [_CTX]),
     "pond_get_fields": (ctypes.c_int, [_CTX, _D, _D, _D, _D, _D, _I32]),
@@ -191,11 +194,11 @@ class Pond:
     def __init__(self, lattice="D3Q27", shape=(16, 16, 16), interp="weno5", precision="fp64", dtau=None,
                  tol_abs=None, tol_rel=None, max_iter=None, use_mlp=False, split_tol_rel=None,
                  split_tol_abs=None, tile_patch=None, refill_chunk=None, pass1_iters=None, global_loads=None,
-                 learned_trust=None, mirror=None):
+                 learned_trust=None, mirror=None, amr=None):
         """Parameters left at None keep pond_params_default's value.  tile_patch, refill_chunk,
         pass1_iters and global_loads set the D3Q27 kernels' work distribution (no effect on
         results); learned_trust the learned start's trust radius; mirror the mirror-boundary axes,
-        bit d for axis d (include/pond.h)."""
+        bit d for axis d; amr the 1D cell activation after every step (include/pond.h)."""
         L = lib()
         p = PondParams()
         L.pond_params_default(ctypes.byref(p), LATTICES[lattice])
@@ -222,6 +225,8 @@ class Pond:
             p.learned_trust = float(learned_trust)
         if mirror is not None:
             p.mirror = int(mirror)
+        if amr is not None:
+            p.amr = int(amr)
         p.use_mlp = 1 if use_mlp else 0
         h = ctypes.c_void_p()
         st = L.pond_create(ctypes.byref(p), ctypes.byref(h))
@@ -272,6 +277,18 @@ class Pond:
         """the current state of another context of the same lattice and grid (pond_copy_state),
         converted to this context's precision on the device"""
         self._check(lib().pond_copy_state(self._h, other._h), "pond_copy_state")
+
+    def active(self):
+        """Cinfo::set of every cell (1 = stepped, 0 = switched off)"""
+        a = np.zeros(self.n, np.int32)
+        self._check(lib().pond_get_active(self._h, a.ctypes.data_as(_I32)), "pond_get_active")
+        return a
+
+    def set_active(self, a):
+        a = np.ascontiguousarray(a, dtype=np.int32).reshape(-1)
+        if a.size != self.n:
+            raise ValueError(f"set_active: {a.size} flags for {self.n} cells")
+        self._check(lib().pond_set_active(self._h, a.ctypes.data_as(_I32)), "pond_set_active")
 
     def step(self, n=1):
         self._check(lib().pond_step(self._h, int(n)), "pond_step")

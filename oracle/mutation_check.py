@@ -7,7 +7,7 @@ a copy of pond_oracle.c with that one change into a temporary directory and runs
 tests/test_oracle_pins.py against it (pyoracle loads the build named by POND_ORACLE_LIB).  A
 mutation the pins do not catch is reported as MISSED and the script exits non-zero.
 
-    python oracle/mutation_check.py
+    python oracle/mutation_check.py [NAME ...]
 """
 import os
 import subprocess
@@ -32,6 +32,27 @@ MUTATIONS = {
     # smoothness row through the departure point instead of (r mod Ny, r mod Nz) (steps.cu:783)
     "weno_row_quirk_off": ("const long by = centred ? wrap(ry, S->p.ny) : wrap(r, S->p.ny);",
                            "const long by = wrap(ry, S->p.ny);"),
+    # generic WENO5: indicator of the middle stencil normalised by 6 (steps.cu:643)
+    "weno5g_beta1_over_6": ("+ 5 * u2 * u4 - 13 * u3 * u4 + 4 * u4 * u4) / 3.0;", "+ 5 * u2 * u4 - 13 * u3 * u4 + 4 * u4 * u4) / 6.0;"),
+    # generic WENO5: linear weight of the left stencil on the right one (steps.cu:621)
+    "weno5g_d0_sign": ("const double d0 = (cdist * cdist + 3 * cdist + 2) / 12.;",
+                       "const double d0 = (cdist * cdist - 3 * cdist + 2) / 12.;"),
+    # generic WENO5: lshift = 0, the commented-out choice (steps.cu:614-615)
+    "weno5g_lshift0": ("const int lshift = 1;", "const int lshift = 0;"),
+    # generic WENO5: weight of S0 from the left stencil's indicator (steps.cu:645)
+    "weno5g_w0_beta2": ("double w0 = d0 / ((1e-6 + beta0) * (1e-6 + beta0));",
+                        "double w0 = d0 / ((1e-6 + beta2) * (1e-6 + beta2));"),
+    # AMR_remove: neighbours up to 4 cells away instead of 3 (amr.cu:50)
+    "amr_remove_bound_4": ("(distM < 4 && distP < 4)", "(distM <= 4 && distP < 4)"),
+    # AMR_add: new cell one past the middle of the gap (amr.cu:15)
+    "amr_add_mid_offset": ("(x + dist / 2) % nx", "(x + dist / 2 + 1) % nx"),
+    # AMR_add: criterion inverted (amr.cu:16)
+    "amr_add_criterion": ("> 0.01 || fabs(1 - rec_rho", "< 0.01 || fabs(1 - rec_rho"),
+    # AMR_add: the new cell's density from one side only (amr.cu:19)
+    "amr_add_one_sided": ("c0.rho = 0.5 * (cm.rho + cp.rho);", "c0.rho = 0.5 * (cm.rho + cm.rho);"),
+    # LBMpull steps switched-off cells (steps.cu:359)
+    "amr_step_ignores_set": ("held (LBMpull, steps.cu:359) */\n  if (!S->set[idx]) return;",
+                             "held (LBMpull, steps.cu:359) */\n  if (0) return;"),
     # D3Q125 interpolation origin half a cell off in y (steps.cu:163,209)
     "q125_origin_half_y": ("crd[1] - (double)(y - 1), crd[2] - (double)(z - 1)};",
                            "crd[1] - (double)(y - 1) + 0.5, crd[2] - (double)(z - 1)};"),
@@ -63,8 +84,11 @@ MUTATIONS = {
 def main():
     src = open(os.path.join(HERE, "pond_oracle.c")).read()
     missed = []
+    only = sys.argv[1:]  # optional subset of mutation names
     with tempfile.TemporaryDirectory() as tmp:
         for name, (a, b) in MUTATIONS.items():
+            if only and name not in only:
+                continue
             if src.count(a) != 1:
                 print(f"{name}: pattern not found exactly once in pond_oracle.c", flush=True)
                 missed.append(name)
@@ -84,7 +108,8 @@ def main():
             print(f"{name}: {'CAUGHT by ' + failed[0] if caught else 'MISSED'}", flush=True)
             if not caught:
                 missed.append(name)
-    print(f"{len(MUTATIONS) - len(missed)}/{len(MUTATIONS)} mutations caught")
+    total = len(only) if only else len(MUTATIONS)
+    print(f"{total - len(missed)}/{total} mutations caught")
     sys.exit(1 if missed else 0)
 
 

@@ -48,6 +48,7 @@
 #define ORC_WENO5 0
 #define ORC_LAGR4 1
 #define ORC_WENO5C 2
+#define ORC_WENO5G 3
 #define QMAX 125
 
 typedef struct {
@@ -66,6 +67,7 @@ typedef struct {
   double dtau, tol_abs, tol_rel;
   int max_iter;
   int mirror;  /* bit d: mirror boundaries on axis d (the reference's outflow switch) */
+  int amr;     /* 1: cell activation and removal along x after every step (amr.cu) */
 } orc_params;
 
 typedef struct {
@@ -76,6 +78,8 @@ typedef struct {
   double* ld;       /* load buffer (state at the start of the step) */
   double* st;       /* store buffer */
   int* niter;
+  int* set;         /* Cinfo::set: 1 = the cell is stepped (structs.cuh:92) */
+  int* setnew;      /* Cinfo::setnew */
 } orc_state;
 
 /* ---------------------------------------------------------------- lattice tables */
@@ -404,6 +408,43 @@ static double interp_weno5_impl(const orc_state* S, const double* crd, int i, co
   return w0 * S0 + w1 * S1 + w2 * S2;
 }
 
+/* the generic propagateWENO<5,M> (steps.cu:610-654) with propagate3 (steps.cu:909-923; the
+   same sum as LagrEval<3>).  lshift = 1 (steps.cu:615), so S0, S1, S2 have x-left nodes r, r-1,
+   r-2: the mirror image of propagateWENO<3,5>'s numbering, with the linear weight
+   (c^2+3c+2)/12 on S0.  y/z nodes floor(y+0.5)+0..2 and the smoothness row (r mod Ny, r mod Nz)
+   as in propagateWENO<3,5> (steps.cu:616-624).  The indicators are written expanded
+   (steps.cu:642-644) and each weight is d / (1e-6 + beta)^2 in one expression (645-647).  It is
+   the same interpolant as propagateWENO<3,5>; the operation order, and so the rounding,
+   differs. */
+static double interp_weno5g(const orc_state* S, const double* crd, int i, const double* base) {
+  const int lshift = 1;
+  const int r = floor_int(crd[0] + 0.5);
+  const int ly = floor_int(crd[1] + 0.5), lz = floor_int(crd[2] + 0.5);
+  const double S0 = lagreval3(S, crd, r - lshift + 1, ly, lz, i, base);
+  const double S1 = lagreval3(S, crd, r - lshift, ly, lz, i, base);
+  const double S2 = lagreval3(S, crd, r - lshift - 1, ly, lz, i, base);
+  const double cdist = crd[0] - r;
+  const double d0 = (cdist * cdist + 3 * cdist + 2) / 12.;
+  const double d2 = (cdist * cdist - 3 * cdist + 2) / 12.;
+  const double d1 = 1. - d0 - d2;
+  /* crd_left = (r - 2, r, r): the samples sit on row y = r, z = r (wrapped) */
+  const long by = wrap(r, S->p.ny), bz = wrap(r, S->p.nz);
+  const double u1 = node_transfer(S, (long)r - lshift - 1, by, bz, base, i);
+  const double u2 = node_transfer(S, (long)r - lshift, by, bz, base, i);
+  const double u3 = node_transfer(S, (long)r - lshift + 1, by, bz, base, i);
+  const double u4 = node_transfer(S, (long)r - lshift + 2, by, bz, base, i);
+  const double u5 = node_transfer(S, (long)r - lshift + 3, by, bz, base, i);
+  const double beta0 = (10 * u3 * u3 - 31 * u3 * u4 + 25 * u4 * u4 + 11 * u3 * u5 - 19 * u4 * u5 + 4 * u5 * u5) / 3.0;
+  const double beta1 = (4 * u2 * u2 - 13 * u2 * u3 + 13 * u3 * u3 + 5 * u2 * u4 - 13 * u3 * u4 + 4 * u4 * u4) / 3.0;
+  const double beta2 = (4 * u1 * u1 - 19 * u1 * u2 + 25 * u2 * u2 + 11 * u1 * u3 - 31 * u2 * u3 + 10 * u3 * u3) / 3.0;
+  double w0 = d0 / ((1e-6 + beta0) * (1e-6 + beta0));
+  double w1 = d1 / ((1e-6 + beta1) * (1e-6 + beta1));
+  double w2 = d2 / ((1e-6 + beta2) * (1e-6 + beta2));
+  const double wsum = w0 + w1 + w2;
+  w0 /= wsum; w1 /= wsum; w2 /= wsum;
+  return w0 * S0 + w1 * S1 + w2 * S2;
+}
+
 /* propagate<4> (steps.cu:986-1036): tricubic Lagrange, nodes floor(x)-1 .. floor(x)+2 */
 static double interp_lagr4(const orc_state* S, const double* crd, int i, const double* base) {
   const int lx = floor_int(crd[0]) - 4 / 2 + 1;
@@ -422,6 +463,7 @@ static double interp_lagr4(const orc_state* S, const double* crd, int i, const d
 
 static double interp_any(const orc_state* S, const double* crd, int i, const double* base) {
   if (S->p.interp == ORC_LAGR4) return interp_lagr4(S, crd, i, base);
+  if (S->p.interp == ORC_WENO5G) return interp_weno5g(S, crd, i, base);
   return interp_weno5_impl(S, crd, i, base, S->p.interp == ORC_WENO5C);
 }
 
@@ -503,6 +545,9 @@ static void collide(const orc_state* S, ocell* c) {
 static void cell_step(orc_state* S, long x, long y, long z) {
   const olat* L = &S->L;
   const long idx = x + S->p.nx * (y + S->p.ny * z);
+  /* a switched-off cell is not stepped: its store record and iteration count keep what they
+     held (LBMpull, steps.cu:359) */
+  if (!S->set[idx]) return;
   ocell c0, cnew;
   load_cell(S, S->ld, idx, &c0);
   zero_reset(L, &c0);
@@ -532,6 +577,7 @@ static void cell_step_q125(orc_state* S, long x, long y, long z) {
   const olat* L = &S->L;
   const int Qn = L->Qn;
   const long idx = x + S->p.nx * (y + S->p.ny * z);
+  if (!S->set[idx]) return; /* steps.cu:156 */
   ocell c0, cnew;
   load_cell(S, S->ld, idx, &c0);
   zero_reset(L, &c0);
@@ -620,17 +666,20 @@ void* orc_create(const orc_params* p) {
   S->ld = (double*)calloc((size_t)S->ncell * S->rec, sizeof(double));
   S->st = (double*)calloc((size_t)S->ncell * S->rec, sizeof(double));
   S->niter = (int*)calloc((size_t)S->ncell, sizeof(int));
-  if (!S->ld || !S->st || !S->niter) {
-    free(S->ld); free(S->st); free(S->niter); free(S);
+  S->set = (int*)calloc((size_t)S->ncell, sizeof(int));
+  S->setnew = (int*)calloc((size_t)S->ncell, sizeof(int));
+  if (!S->ld || !S->st || !S->niter || !S->set || !S->setnew) {
+    free(S->ld); free(S->st); free(S->niter); free(S->set); free(S->setnew); free(S);
     return NULL;
   }
+  for (long n = 0; n < S->ncell; n++) S->set[n] = 1; /* fill() activates every cell (init.cu:106) */
   return S;
 }
 
 void orc_destroy(void* h) {
   orc_state* S = (orc_state*)h;
   if (!S) return;
-  free(S->ld); free(S->st); free(S->niter); free(S);
+  free(S->ld); free(S->st); free(S->niter); free(S->set); free(S->setnew); free(S);
 }
 
 /* init()/fill() analogue (init.cu:96-106): set_base of the given fields into both buffers */
@@ -660,7 +709,68 @@ void orc_set_populations(void* h, const double* f, const double* rho, const doub
   }
 }
 
-/* calcStep() without I/O: one PonD step per iteration, then swap load/store
+/* ---------------------------------------------------------------- 1D cell activation (amr.cu) */
+
+static double rec_rho(const orc_state* S, const double* buf, long idx) { return buf[(size_t)idx * S->rec + S->L.Qn]; }
+static double rec_u(const orc_state* S, const double* buf, long idx, int k) {
+  return buf[(size_t)idx * S->rec + S->L.Qn + 1 + k];
+}
+
+/* AMR_add (amr.cu:1-27), for one active cell: the next active cell along x (wrapping); where
+   their new states differ (|du_x|^2 / min(T) > 0.01 or |1 - rho_m/rho_p| > 0.01) and the gap is
+   even, the cell half way receives the average of the two records in both buffers (save_12)
+   and is switched on from the next step (setnew).  Reads only active cells' store records and
+   `set`, writes only cells strictly inside the gap, so the cells can be taken in any order. */
+static void amr_add_cell(orc_state* S, long x, long y, long z) {
+  const long nx = S->p.nx;
+  const long row = S->p.nx * (y + S->p.ny * z);
+  if (!S->set[row + x]) return;
+  long xp = x, dist = 0;
+  do {
+    xp = (xp + 1) % nx;
+    dist++;
+  } while (!S->set[row + xp]);
+  const long im = row + x, ip = row + xp;
+  const double vdif = rec_u(S, S->st, ip, 0) - rec_u(S, S->st, im, 0);
+  const double Tm = rec_u(S, S->st, im, 3), Tp = rec_u(S, S->st, ip, 3);
+  if (vdif * vdif / fmin(Tm, Tp) > 0.01 || fabs(1 - rec_rho(S, S->st, im) / rec_rho(S, S->st, ip)) > 0.01) {
+    if (dist % 2 == 0) {
+      const long imid = row + (x + dist / 2) % nx;
+      ocell cm, cp, c0;
+      load_cell(S, S->st, im, &cm);
+      load_cell(S, S->st, ip, &cp);
+      c0.rho = 0.5 * (cm.rho + cp.rho);
+      for (int k = 0; k < 4; k++) c0.uT[k] = 0.5 * (cm.uT[k] + cp.uT[k]);
+      for (int i = 0; i < S->L.Qn; i++) c0.f[i] = 0.5 * (cm.f[i] + cp.f[i]);
+      save_cell(S, S->ld, imid, &c0);
+      save_cell(S, S->st, imid, &c0);
+      S->setnew[imid] = 1;
+    }
+  }
+}
+
+/* AMR_remove (amr.cu:28-55): one pass along the row y = z = 0 in increasing x, each decision
+   seeing the ones before it: an active cell whose previous and next active cells lie within 3
+   cells and match it (|du_x|^2 / T < 0.01, |rho ratio - 1| < 0.01 both ways, its own T and rho)
+   is switched off. */
+static void amr_remove_row(orc_state* S) {
+  const long nx = S->p.nx;
+  for (long x = 0; x < nx; x++) {
+    if (!S->set[x]) continue;
+    long xm = x, xp = x, distM = 0, distP = 0;
+    do { xm = (xm - 1 + nx) % nx; distM++; } while (!S->set[xm]);
+    do { xp = (xp + 1) % nx; distP++; } while (!S->set[xp]);
+    const double u0 = rec_u(S, S->st, x, 0), T0 = rec_u(S, S->st, x, 3), r0 = rec_rho(S, S->st, x);
+    const double vdifm = rec_u(S, S->st, xm, 0) - u0;
+    const double vdifp = rec_u(S, S->st, xp, 0) - u0;
+    if (vdifm * vdifm / T0 < 0.01 && vdifp * vdifp / T0 < 0.01 && fabs(rec_rho(S, S->st, xp) / r0 - 1) < 0.01 &&
+        fabs(rec_rho(S, S->st, xm) / r0 - 1) < 0.01 && (distM < 4 && distP < 4))
+      S->set[x] = 0;
+  }
+}
+
+/* calcStep() without I/O: one PonD step per iteration, then (with amr) the activation passes of
+   update.cu:102-105 as they stand with their launches enabled, then swap load/store
    (update.cu:86-142, structs.cuh:148) */
 void orc_step(void* h, int nsteps) {
   orc_state* S = (orc_state*)h;
@@ -673,8 +783,25 @@ void orc_step(void* h, int nsteps) {
       else cell_step(S, x, y, z);
     }
     (void)nz;
+    if (S->p.amr) {
+      for (long n = 0; n < S->ncell; n++) S->setnew[n] = S->set[n]; /* cinfo_setnew_reset */
+      for (long n = 0; n < S->ncell; n++) amr_add_cell(S, n % nx, (n / nx) % ny, n / (nx * ny));
+      for (long n = 0; n < S->ncell; n++) S->set[n] = S->setnew[n]; /* cinfo_set_new */
+      amr_remove_row(S);
+    }
     double* t = S->ld; S->ld = S->st; S->st = t;
   }
+}
+
+/* Cinfo::set of every cell */
+void orc_get_active(void* h, int* set) {
+  orc_state* S = (orc_state*)h;
+  for (long n = 0; n < S->ncell; n++) set[n] = S->set[n];
+}
+
+void orc_set_active(void* h, const int* set) {
+  orc_state* S = (orc_state*)h;
+  for (long n = 0; n < S->ncell; n++) S->set[n] = set[n] != 0;
 }
 
 /* fields of the current state (the last step's output, i.e. the load buffer after the swap) */
@@ -706,7 +833,8 @@ double orc_interp(void* h, int i, const double* crd, const double* g) {
   return interp_any(S, crd, i, g);
 }
 
-/* calcTotMoments (update.cu:52-69), summed serially in cell order; diagn k (diagn.cu:3-22) */
+/* calcTotMoments (update.cu:52-69, active cells), summed serially in cell order; diagn k
+   (diagn.cu:3-22, every cell) */
 void orc_totals(void* h, double* out4) {
   orc_state* S = (orc_state*)h;
   const int Qn = S->L.Qn;
@@ -717,9 +845,11 @@ void orc_totals(void* h, double* out4) {
   for (long n = 0; n < S->ncell; n++) {
     const double* r = S->ld + (size_t)n * S->rec;
     const double rho = r[Qn], ux = r[Qn + 1], uy = r[Qn + 2], uz = r[Qn + 3], T = r[Qn + 4];
-    mass += rho * 1.0;
-    mom += rho * ux * 1.0;
-    en += rho * (ux * ux + uy * uy + uz * uz + Dim * T) * 1.0;
+    if (S->set[n]) {
+      mass += rho * 1.0;
+      mom += rho * ux * 1.0;
+      en += rho * (ux * ux + uy * uy + uz * uz + Dim * T) * 1.0;
+    }
     k += 0.5 * d8pi3 * (ux * ux + uy * uy + uz * uz) * dx * dy * dz;
   }
   out4[0] = mass; out4[1] = mom; out4[2] = en; out4[3] = k;

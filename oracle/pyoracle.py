@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # tests against deliberately broken builds that way)
 LIB_PATH = os.environ.get("POND_ORACLE_LIB") or os.path.join(_HERE, "_build", "libpond_oracle.so")
 LATTICES = {"D1Q5": 0, "D3Q27": 1, "D3Q125": 2}
-INTERPS = {"weno5": 0, "lagr4": 1, "weno5c": 2}
+INTERPS = {"weno5": 0, "lagr4": 1, "weno5c": 2, "weno5g": 3}
 QN = {"D1Q5": 5, "D3Q27": 27, "D3Q125": 125}
 
 _D = ctypes.POINTER(ctypes.c_double)
@@ -34,6 +34,7 @@ class OrcParams(ctypes.Structure):
         ("tol_rel", ctypes.c_double),
         ("max_iter", ctypes.c_int),
         ("mirror", ctypes.c_int),
+        ("amr", ctypes.c_int),
     ]
 
 
@@ -74,6 +75,10 @@ def lib():
         L.orc_get_populations.argtypes = [ctypes.c_void_p, _D]
         L.orc_interp.restype = ctypes.c_double
         L.orc_interp.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D]
+        L.orc_get_active.restype = None
+        L.orc_get_active.argtypes = [ctypes.c_void_p, _I32]
+        L.orc_set_active.restype = None
+        L.orc_set_active.argtypes = [ctypes.c_void_p, _I32]
         L.orc_totals.restype = None
         L.orc_totals.argtypes = [ctypes.c_void_p, _D]
         L.orc_threads.restype = ctypes.c_int
@@ -164,11 +169,12 @@ class Oracle:
     """CPU restatement of the reference step on one grid."""
 
     def __init__(self, lattice, shape, interp="weno5", dtau=1.0 / 0.9, tol_abs=1e-12, tol_rel=1e-10, max_iter=100,
-                 mirror=0):
+                 mirror=0, amr=0):
         """mirror: bit d set = mirror boundaries on axis d (Lagrange-4 and D3Q125 only, as in the
-        reference, whose WENO5 interpolation has no mirror path)"""
+        reference, whose WENO5 interpolation has no mirror path).  amr: 1 = the 1D cell activation
+        and removal passes (amr.cu) after every step."""
         p = OrcParams(LATTICES[lattice], INTERPS[interp], int(shape[0]), int(shape[1]), int(shape[2]), float(dtau),
-                      float(tol_abs), float(tol_rel), int(max_iter), int(mirror))
+                      float(tol_abs), float(tol_rel), int(max_iter), int(mirror), int(amr))
         self._h = lib().orc_create(ctypes.byref(p))
         if not self._h:
             raise MemoryError("orc_create failed")
@@ -198,6 +204,16 @@ class Oracle:
 
     def step(self, n=1):
         lib().orc_step(self._h, int(n))
+
+    def active(self):
+        """Cinfo::set of every cell (1 = stepped)"""
+        a = np.zeros(self.n, np.int32)
+        lib().orc_get_active(self._h, a.ctypes.data_as(_I32))
+        return a
+
+    def set_active(self, a):
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        lib().orc_set_active(self._h, a.ctypes.data_as(_I32))
 
     def fields(self):
         out = {k: np.zeros(self.n) for k in ("rho", "ux", "uy", "uz", "T")}

@@ -100,3 +100,19 @@ def test_arr_reader_reads_reference_layout(tmp_path):
     arr = lbmpond_amd.read_arr(str(path))
     assert arr.shape == (nz, ny, nx)
     assert np.array_equal(arr.reshape(-1), data)
+
+
+def test_amr_is_limited_to_fp64_d1q5_lines():
+    """the 1D cell activation (amr.cu) is accepted for POND_FP64 D1Q5 lines only, and the
+    removal pass stages the line in LDS, so its length is bounded"""
+    L = lbmpond_amd.lib()
+    cases = [(1, 0, (64, 4, 4)), (0, 0, (64, 2, 1)), (0, 0, (40000, 1, 1)), (0, 0, (64, 1, 1))]
+    for lat, prec, shape in cases:
+        p = pondmod.PondParams()
+        L.pond_params_default(ctypes.byref(p), lat)
+        p.precision = prec
+        p.nx, p.ny, p.nz = shape
+        p.amr = 1 if shape != (64, 1, 1) else 2
+        h = ctypes.c_void_p()
+        assert L.pond_create(ctypes.byref(p), ctypes.byref(h)) == 1, (lat, shape)
+        assert b"amr" in L.pond_last_error(None)

@@ -214,7 +214,7 @@ def uniform_gauge_state(lat, shape, F, gauge):
     return f, mom
 
 
-@pytest.mark.parametrize("interp", ["weno5", "weno5c"])
+@pytest.mark.parametrize("interp", ["weno5", "weno5c", "weno5g"])
 def test_weno5_quartic_with_flat_smoothness_row(interp):
     """F = 1 + g(y,z) p(x): p quartic in x, g quadratic in y,z and zero on the smoothness row.
     All three sub-stencils then see equal (zero) smoothness indicators, the nonlinear weights
@@ -227,7 +227,7 @@ def test_weno5_quartic_with_flat_smoothness_row(interp):
     for trial in range(6):
         crd = np.array([11.0, 10.0, 12.0]) + rng.uniform(-0.49, 0.49, 3)
         r = int(math.floor(crd[0] + 0.5))
-        if interp == "weno5":
+        if interp in ("weno5", "weno5g"):
             by, bz = r % ny, r % nz
         else:
             by, bz = int(math.floor(crd[1] + 0.5)), int(math.floor(crd[2] + 0.5))
@@ -293,6 +293,7 @@ def test_moments_of_set_base(lat):
 
 @pytest.mark.parametrize("lat,interp,shape", [("D1Q5", "weno5", (24, 1, 1)), ("D1Q5", "lagr4", (24, 1, 1)),
                                               ("D3Q27", "weno5", (6, 5, 4)), ("D3Q27", "weno5c", (6, 5, 4)),
+                                              ("D3Q27", "weno5g", (6, 5, 4)), ("D1Q5", "weno5g", (24, 1, 1)),
                                               ("D3Q27", "lagr4", (6, 5, 4)), ("D3Q125", "weno5", (3, 4, 3))])
 def test_uniform_equilibrium_is_fixed_point(lat, interp, shape):
     n = int(np.prod(shape))
@@ -389,7 +390,7 @@ def weno5_numpy(u, c):
     return sum(w[k] * S[k] for k in range(3)) / sum(w)
 
 
-@pytest.mark.parametrize("interp", ["weno5", "weno5c"])
+@pytest.mark.parametrize("interp", ["weno5", "weno5c", "weno5g"])
 def test_weno5_nonlinear_weights_match_jiang_shu(interp):
     """A field varying only along x, with a steep front, so the three smoothness indicators
     differ by orders of magnitude and the nonlinear weights are far from the linear ones: the
@@ -423,7 +424,35 @@ def test_weno5_nonlinear_weights_match_jiang_shu(interp):
     assert max(spread) > 1e-3  # the test exercises strongly nonlinear weights
 
 
-@pytest.mark.parametrize("interp,left", [("weno5", 0), ("weno5c", -1)])
+def test_generic_weno5_is_the_same_interpolant_in_another_order():
+    """propagateWENO<5,M> (steps.cu:610-654) numbers the sub-stencils from the right and writes
+    the indicators expanded; mathematically it is propagateWENO<3,5>.  On a random 3D field (all
+    indicators different, nonlinear weights far from the linear ones) the two agree to round-off
+    at every departure point, and differ in the last bits at some: they are two operation orders
+    of one interpolant, not one code path under two names."""
+    shape = (12, 9, 8)
+    n = int(np.prod(shape))
+    rng = np.random.default_rng(21)
+    q = po.QN["D3Q27"]
+    w = po.lattice("D3Q27")[2]
+    base = rng.uniform(0.5, 1.5, n)
+    f = np.outer(w, base) * rng.uniform(0.6, 1.4, (q, n))
+    gauge = np.array([0.02, -0.03, 0.01, 0.3])
+    mom = [base, np.full(n, 0.01), np.full(n, -0.02), np.full(n, 0.015), np.full(n, 0.3)]
+    a, b = po.Oracle("D3Q27", shape, "weno5"), po.Oracle("D3Q27", shape, "weno5g")
+    a.set_populations(f, *mom)
+    b.set_populations(f, *mom)
+    differ = 0
+    for _ in range(200):
+        crd = rng.uniform([0, 0, 0], shape)
+        i = int(rng.integers(0, q))
+        va, vb = a.interp(i, crd, gauge), b.interp(i, crd, gauge)
+        assert abs(va - vb) <= 1e-13 * max(abs(va), 1e-300), (crd, i, va, vb)
+        differ += va != vb
+    assert differ > 0
+
+
+@pytest.mark.parametrize("interp,left", [("weno5", 0), ("weno5c", -1), ("weno5g", 0)])
 def test_weno5_yz_substencil_nodes_on_a_cubic(interp, left):
     """F = 1 + a (y - y0)^3 + b (z - z0)^2, flat in x: every smoothness indicator is zero (so the
     weights are the linear ones, which sum to one) and the interpolant is the y/z quadratic
@@ -551,6 +580,25 @@ def test_reverse_tables():
         assert (r[d][~moved] == np.arange(125)[~moved]).all()
 
 
+@pytest.mark.parametrize("lattice", ["D1Q5", "D3Q27", "D3Q125"])
+def test_mirrored_cell_equals_reversed_load(lattice):
+    """Two forms of the mirror in the reference: PonDstepD3Q125 loads population reverseX[i]
+    into slot i (steps.cu:180-184); PonDstepD3Q125_iterative (steps.cu:251-355, otherwise the
+    same kernel line for line) loads the cell as it is and applies Cell::mirrX (structs.cuh:
+    552-578), which keeps f[i] where e_x = 0 and takes f[reverseX[i]] elsewhere.  They stage the
+    same cell exactly when every table fixes the populations with a zero component on its axis
+    (the oracle uses the mirrX form, the HIP kernels the reversed loads, and they agree bit for
+    bit with mirror walls)."""
+    _, ef, _, _, _ = po.lattice(lattice)
+    rev = po.reverse(lattice)
+    q = ef.shape[0]
+    rng = np.random.default_rng(3)
+    f = rng.uniform(0.1, 1.0, q)
+    for d in range(3 if lattice != "D1Q5" else 1):
+        mirr = np.where(ef[:, d] == 0, f, f[rev[d]])
+        assert np.array_equal(mirr, f[rev[d]])
+
+
 def _mirror_image(lattice, f, fields, shape, axis):
     """state and populations on the domain doubled along `axis` (mirror image in the second half)"""
     nx, ny, nz = shape
@@ -613,3 +661,84 @@ def test_mirror_walls_equal_the_doubled_periodic_domain(lattice, shape, axes, in
         p.set_populations(f, *fields)
         p.step(steps)
         assert np.abs(p.fields()["rho"] - ra["rho"]).max() > 1e-6
+
+
+# ----------------------------------------------------------------------------- 1D cell activation
+
+def _line(n, rho, u=0.05, T=0.02):
+    z = np.zeros(n)
+    return np.asarray(rho, float), np.full(n, u), z, z, np.full(n, T)
+
+
+@pytest.mark.parametrize("nx", [40, 64])
+def test_amr_remove_is_one_ordered_pass(nx):
+    """On a uniform line every cell matches its neighbours, so AMR_remove (amr.cu:28-55) switches
+    off every cell whose previous active cell lies within 3: taken in increasing x, each decision
+    seeing the earlier ones, that leaves exactly the cells x = 3 mod 4 (cell 3 sees 2, 1, 0 off and
+    its previous active cell 4 away).  Simultaneous decisions, a bound of <= 4 or a pass in the
+    other direction give other patterns.  AMR_add then finds nothing to add (gap 4, equal states)
+    and the pattern is stable."""
+    o = po.Oracle("D1Q5", (nx, 1, 1), "lagr4", amr=1)
+    o.set_state(*_line(nx, np.full(nx, 1.2)))
+    o.step(1)
+    want = (np.arange(nx) % 4 == 3).astype(np.int32)
+    assert np.array_equal(o.active(), want)
+    o.step(3)
+    assert np.array_equal(o.active(), want)
+
+
+def test_switched_off_cells_are_not_stepped():
+    """LBMpull returns at once for a cell with set == 0 (steps.cu:359): its records in both
+    buffers stay as they were (both hold the initial state after fill), whatever its neighbours
+    do, while the active cells evolve; a switched-off cell is still an interpolation node."""
+    n = 48
+    f = smooth_random_state((n, 1, 1), seed=31, T0=0.03, dims=1)
+    act = np.ones(n, np.int32)
+    act[[5, 6, 20, 33]] = 0
+    o = po.Oracle("D1Q5", (n, 1, 1), "lagr4")
+    o.set_state(*f)
+    f0 = o.fields()
+    o.set_active(act)
+    for _ in range(3):
+        o.step(1)
+        F = o.fields()
+        off = act == 0
+        for k, a in zip(("rho", "ux", "T"), (f0["rho"], f0["ux"], f0["T"])):
+            assert np.array_equal(F[k][off], a[off])
+            assert not np.array_equal(F[k][~off], a[~off])
+    # the stepped cells next to the frozen ones differ from a run with every cell active
+    ref = po.Oracle("D1Q5", (n, 1, 1), "lagr4")
+    ref.set_state(*f)
+    ref.step(3)
+    assert not np.array_equal(ref.fields()["rho"][[4, 7]], o.fields()["rho"][[4, 7]])
+
+
+def test_amr_add_fills_the_middle_of_a_gap_across_a_jump():
+    """With every fourth cell active (3 mod 4), a density jump between two active cells makes
+    AMR_add (amr.cu:1-27) switch on the cell half way (gap 4 -> offset 2) with the average of the
+    two new records in both buffers; gaps over the smooth parts (a density slope of 0.2 % per
+    gap, under the 1 % criterion) get nothing, so their cells keep their records (a cell added
+    there and switched off again by AMR_remove in the same step would hold an average)."""
+    n = 64
+    x = np.arange(n)
+    rho = np.where(x < 33, 1.0, 1.25) * (1 + 0.0005 * x)
+    o = po.Oracle("D1Q5", (n, 1, 1), "lagr4", amr=1)
+    o.set_state(*_line(n, rho, u=0.0, T=0.02))
+    pattern = (np.arange(n) % 4 == 3).astype(np.int32)
+    o.set_active(pattern)
+    f0 = o.fields()
+    o.step(1)
+    F = o.fields()
+    act = o.active()
+    # jumps between active cells: 31 | 35 (the step has spread the jump into both) and the
+    # periodic wrap 63 | 3
+    added = np.flatnonzero(act & ~pattern.astype(bool))
+    assert set(added.tolist()) == {33, 1}, added
+    for mid, (a, b) in {33: (31, 35), 1: (63, 3)}.items():
+        for k in ("rho", "ux", "T"):
+            assert F[k][mid] == 0.5 * (F[k][a] + F[k][b]), (mid, k)
+    # nothing switched off: every active cell has a neighbour 4 away
+    assert (act[pattern == 1] == 1).all()
+    untouched = (pattern == 0) & ~np.isin(x, [33, 1])
+    for k in ("rho", "ux", "T"):
+        assert np.array_equal(F[k][untouched], f0[k][untouched]), k

@@ -83,6 +83,33 @@ def test_d3q27_weno5_as_written_15_steps(gpu):
     check(ref, got, fr, fg)
 
 
+def test_d1q5_sod_generic_weno5_100_steps(gpu):
+    """propagateWENO<5,M> (the generic template, sub-stencils numbered from the right) on the
+    1D Sod tube, bit for bit against the oracle; its rounding differs from propagateWENO<3,5>
+    (the two are compared as well, to show that the kernel really is the other operation
+    order: equal to 1e-9, not bitwise)."""
+    case = ic.sod_1d(2000)
+    ref, got, fr, fg = run_pair("D1Q5", case.shape, "weno5g", case.fields, 100, dtau=case.dtau)
+    check(ref, got, fr, fg)
+    with Pond("D1Q5", case.shape, "weno5", "fp64", dtau=case.dtau) as p:
+        p.set_state(*case.fields)
+        p.step(100)
+        other = p.fields()
+    assert rel_linf(got["rho"], other["rho"]) < 1e-9
+    assert not np.array_equal(got["rho"], other["rho"])
+
+
+def test_d3q27_generic_weno5_15_steps(gpu):
+    """the generic WENO5 in 3D, from an x-dominant state (its y/z stencil extrapolates downwind
+    like propagateWENO<3,5>'s), bit for bit against the oracle, on the production tile order
+    (64 x 8 x 4: row patches and whole-row lane refill) and on a ragged grid"""
+    for shape, seed in (((8, 6, 5), 25), ((64, 8, 4), 26)):
+        f = x_dominant_state(shape, seed=seed, T0=0.01)
+        ref, got, fr, fg = run_pair("D3Q27", shape, "weno5g", f, 15)
+        assert np.isfinite(ref["rho"]).all()
+        check(ref, got, fr, fg)
+
+
 def test_d3q27_weno5c_ragged_grid(gpu):
     # N not a multiple of the 64-lane wave; rows shorter than a wave; Ny != Nz
     shape = (13, 9, 7)

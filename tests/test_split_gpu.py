@@ -7,7 +7,7 @@ measured error and the mean gauge iterations of both paths.
 import numpy as np
 import pytest
 
-from helpers import STATE_ERRORS, rel_linf, smooth_random_state
+from helpers import FIELDS, STATE_ERRORS, rel_linf, smooth_random_state, x_dominant_state
 from lbmpond_amd import Pond
 from lbmpond_amd.compare import CapDependence, state_errors
 from lbmpond_amd import ic
@@ -54,6 +54,17 @@ def compare(shape, interp, fields, steps, tol, dtau=1.0 / 0.9, lattice="D3Q27", 
 def test_split_weno5c_random(gpu):
     shape = (16, 12, 10)
     compare(shape, "weno5c", smooth_random_state(shape, seed=61, T0=0.01), 20, 1e-5)
+
+
+def test_split_generic_weno5_runs_the_as_written_kernel(gpu):
+    """POND_WENO5G in the split path is POND_WENO5's kernel (the same interpolant; the split
+    arithmetic does not keep the FP64 operation order anyway): bitwise equal results"""
+    shape = (16, 8, 4)
+    f = x_dominant_state(shape, seed=62, T0=0.01)
+    a = run("D3Q27", shape, "weno5g", "split", f, 3)
+    b = run("D3Q27", shape, "weno5", "split", f, 3)
+    for k in FIELDS:
+        assert np.array_equal(a[k], b[k]), k
 
 
 def test_split_lagr4_random(gpu):
