@@ -152,6 +152,7 @@ def run_path(case, interp, name, collect, warmup, steps, train_cfg, hooks=None, 
     r.p.set_state(*case.fields)
     total = collect + warmup + steps
     t0 = r.p.timing()
+    last_log = time.time()
     for s in range(total):
         if hooks:
             hooks(s, r)
@@ -161,8 +162,9 @@ def run_path(case, interp, name, collect, warmup, steps, train_cfg, hooks=None, 
         r.step(phase, s, timed=phase == "timed")
         if phase == "collect" and s == collect - 1 and r.learned:
             r.train()
-        if total <= 40 or (s + 1) % 100 == 0 or s + 1 == total:
+        if total <= 40 or (s + 1) % 100 == 0 or s + 1 == total or time.time() - last_log > 30:
             log(f"[bench] {name}: {phase} step {s + 1}/{total}")
+            last_log = time.time()
     t1 = r.p.timing()
     cells = int(np.prod(case.shape))
     r.res.update(cells=cells, wall_s=r.wall, kernel_ms=t1["total_ms"] - t0["total_ms"], mlp_ms=t1["last_mlp_ms"],

@@ -409,6 +409,47 @@ __device__ __forceinline__ void lag4_factors(double s, double F[4][3]) {
   }
 }
 
+// the WENO5 combination of the three sub-stencil sums (S0: left window, S2: right) with the
+// smoothness samples ub at the five window positions, in the operation order of form VAR
+template <int VAR>
+__device__ __forceinline__ double weno5_combine_f64(double cdist, const double (&ub)[5], double S0, double S1,
+                                                    double S2) {
+  const double u1 = ub[0], u2 = ub[1], u3 = ub[2], u4 = ub[3], u5 = ub[4];
+  if (VAR == 2) {
+    // S0, S1, S2 of steps.cu:617-619 are this function's right, middle and left sums
+    const double d0 = (cdist * cdist + 3 * cdist + 2) / 12.;
+    const double d2 = (cdist * cdist - 3 * cdist + 2) / 12.;
+    const double d1 = 1. - d0 - d2;
+    const double beta0 = (10 * u3 * u3 - 31 * u3 * u4 + 25 * u4 * u4 + 11 * u3 * u5 - 19 * u4 * u5 + 4 * u5 * u5) / 3.0;
+    const double beta1 = (4 * u2 * u2 - 13 * u2 * u3 + 13 * u3 * u3 + 5 * u2 * u4 - 13 * u3 * u4 + 4 * u4 * u4) / 3.0;
+    const double beta2 = (4 * u1 * u1 - 19 * u1 * u2 + 25 * u2 * u2 + 11 * u1 * u3 - 31 * u2 * u3 + 10 * u3 * u3) / 3.0;
+    double w0 = d0 / ((1e-6 + beta0) * (1e-6 + beta0));
+    double w1 = d1 / ((1e-6 + beta1) * (1e-6 + beta1));
+    double w2 = d2 / ((1e-6 + beta2) * (1e-6 + beta2));
+    const double wsum = w0 + w1 + w2;
+    w0 /= wsum;
+    w1 /= wsum;
+    w2 /= wsum;
+    return w0 * S2 + w1 * S1 + w2 * S0;
+  }
+  const double d0 = (cdist * cdist - 3 * cdist + 2) / 12.;
+  const double d2 = (cdist * cdist + 3 * cdist + 2) / 12.;
+  const double d1 = 1. - d0 - d2;
+  const double beta0 = (10 * u3 * u3 + (11 * u1 - 31 * u2) * u3 + 25 * u2 * u2 - 19 * u1 * u2 + 4 * u1 * u1) / 3.0;
+  const double beta1 = (4 * u4 * u4 + (5 * u2 - 13 * u3) * u4 + 13 * u3 * u3 - 13 * u2 * u3 + 4 * u2 * u2) / 3.0;
+  const double beta2 = (4 * u5 * u5 + (11 * u3 - 19 * u4) * u5 + 25 * u4 * u4 - 31 * u3 * u4 + 10 * u3 * u3) / 3.0;
+  double w0 = d0 * 1, w1 = d1 * 1, w2 = d2 * 1;
+  const double sc = 1e-6;
+  w0 /= (sc + beta0) * (sc + beta0);
+  w1 /= (sc + beta1) * (sc + beta1);
+  w2 /= (sc + beta2) * (sc + beta2);
+  const double wsum = w0 + w1 + w2;
+  w0 /= wsum;
+  w1 /= wsum;
+  w2 /= wsum;
+  return w0 * S0 + w1 * S1 + w2 * S2;
+}
+
 // WENO5 interpolation, three forms (VAR):
 // 0: propagateWENO<3,5> (steps.cu:774-825) with LagrEval<3> (steps.cu:924-937), as written:
 //    y/z sub-stencil nodes floor(y+0.5)+0..2 and smoothness samples on the row
@@ -492,41 +533,184 @@ __device__ double interp_weno5(const SRC& src, double cx, double cy, double cz, 
         for (int b = 0; b < 3; b++) S2 += p * Fy[a][0] * Fy[a][1] * Fz[b][0] * Fz[b][1] * T[3 * a + b];
     }
   }
-  const double cdist = cx - r;
-  const double u1 = ub[0], u2 = ub[1], u3 = ub[2], u4 = ub[3], u5 = ub[4];
-  if (VAR == 2) {
-    // S0, S1, S2 of steps.cu:617-619 are this function's right, middle and left sums
-    const double d0 = (cdist * cdist + 3 * cdist + 2) / 12.;
-    const double d2 = (cdist * cdist - 3 * cdist + 2) / 12.;
-    const double d1 = 1. - d0 - d2;
-    const double beta0 = (10 * u3 * u3 - 31 * u3 * u4 + 25 * u4 * u4 + 11 * u3 * u5 - 19 * u4 * u5 + 4 * u5 * u5) / 3.0;
-    const double beta1 = (4 * u2 * u2 - 13 * u2 * u3 + 13 * u3 * u3 + 5 * u2 * u4 - 13 * u3 * u4 + 4 * u4 * u4) / 3.0;
-    const double beta2 = (4 * u1 * u1 - 19 * u1 * u2 + 25 * u2 * u2 + 11 * u1 * u3 - 31 * u2 * u3 + 10 * u3 * u3) / 3.0;
-    double w0 = d0 / ((1e-6 + beta0) * (1e-6 + beta0));
-    double w1 = d1 / ((1e-6 + beta1) * (1e-6 + beta1));
-    double w2 = d2 / ((1e-6 + beta2) * (1e-6 + beta2));
-    const double wsum = w0 + w1 + w2;
-    w0 /= wsum;
-    w1 /= wsum;
-    w2 /= wsum;
-    return w0 * S2 + w1 * S1 + w2 * S0;
+  return weno5_combine_f64<VAR>(cx - r, ub, S0, S1, S2);
+}
+
+// ---------------------------------------------------------------------------------------------
+// D3Q27 targets in (ey, ez) groups: the three targets with ex = -1, 0, +1 share their departure
+// point's y and z (v_y and v_z depend on ey, ez only), so their stencils share the y/z rows and
+// differ in x only.  A node of the union of their windows is loaded once and transferred to each
+// target whose window holds it, with exactly transfer27's operations for that target; the y and
+// z basis values, which the three share, are evaluated once.  Each target's interpolation sums
+// receive their terms in the order of the one-target code (slices in increasing x, rows (a, b)
+// in order inside a slice), so every value is bitwise what interp_weno5 / interp_lagr4 give.
+
+// The three targets of a group: the fields of Tgt27 they share (u1, T1 and the y, z target
+// components) once, the x ones per target.
+struct Tgt27g3 {
+  double u1x, u1y, u1z, T1, Cy, Cz;
+  bool ky, kz;
+  double W[3], Cx[3];
+  bool kx[3], wok[3];
+};
+
+// transfer27 of one node to the targets k with use[k] (T[k]; the others untouched)
+template <class SRC>
+__device__ __forceinline__ void transfer27_g3(const SRC& s, uint32_t boff, const Tgt27g3& t, const bool (&use)[3],
+                                              double sqTb, double (&T)[3]) {
+  double u0x, u0y, u0z, Pn;
+  node_g4(s, boff, u0x, u0y, u0z, Pn);
+  const bool nok = node_ok(s, boff);
+  // basis27u, split into the x part (per target) and the shared y and z parts
+  const double Dx = (t.u1x - u0x) * sqTb;
+  const double Dy = (t.u1y - u0y) * sqTb;
+  const double Dz = (t.u1z - u0z) * sqTb;
+  const double E[3] = {-Pn, 0.0 * Pn, Pn};
+  double ax[3], gy[3], gz[3];
+  bool syz = true;
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    ax[j] = Dx - E[j];
+    const double ay = Dy - E[j], az = Dz - E[j];
+    gy[j] = ay * ay - (t.ky ? t.Cy * ay : t.T1);
+    gz[j] = az * az - (t.kz ? t.Cz * az : t.T1);
+    syz = syz && fabs(gy[j]) < kSafe && fabs(gz[j]) < kSafe;
   }
-  const double d0 = (cdist * cdist - 3 * cdist + 2) / 12.;
-  const double d2 = (cdist * cdist + 3 * cdist + 2) / 12.;
-  const double d1 = 1. - d0 - d2;
-  const double beta0 = (10 * u3 * u3 + (11 * u1 - 31 * u2) * u3 + 25 * u2 * u2 - 19 * u1 * u2 + 4 * u1 * u1) / 3.0;
-  const double beta1 = (4 * u4 * u4 + (5 * u2 - 13 * u3) * u4 + 13 * u3 * u3 - 13 * u2 * u3 + 4 * u2 * u2) / 3.0;
-  const double beta2 = (4 * u5 * u5 + (11 * u3 - 19 * u4) * u5 + 25 * u4 * u4 - 31 * u3 * u4 + 10 * u3 * u3) / 3.0;
-  double w0 = d0 * 1, w1 = d1 * 1, w2 = d2 * 1;
-  const double sc = 1e-6;
-  w0 /= (sc + beta0) * (sc + beta0);
-  w1 /= (sc + beta1) * (sc + beta1);
-  w2 /= (sc + beta2) * (sc + beta2);
-  const double wsum = w0 + w1 + w2;
-  w0 /= wsum;
-  w1 /= wsum;
-  w2 /= wsum;
-  return w0 * S0 + w1 * S1 + w2 * S2;
+  double f[27];
+  node_f27(s, boff, f);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    if (!use[k]) continue;
+    double gx[3];
+    bool safe = nok && t.wok[k] && syz;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      gx[j] = ax[j] * ax[j] - (t.kx[k] ? t.Cx[k] * ax[j] : t.T1);
+      safe = safe && fabs(gx[j]) < kSafe;
+    }
+    double p2[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) p2[a][b] = t.W[k] * gx[a] * gy[b];
+    double fnew = 0;
+#pragma unroll
+    for (int iold = 0; iold < 27; iold++) {
+      const int jx = kE27[iold][0] + 1, jy = kE27[iold][1] + 1, jz = kE27[iold][2] + 1;
+      fnew += p2[jx][jy] * gz[jz] * f[iold];
+    }
+    if (!safe) {
+      fnew = 0;
+#pragma unroll
+      for (int iold = 0; iold < 27; iold++) {
+        const int jx = kE27[iold][0] + 1, jy = kE27[iold][1] + 1, jz = kE27[iold][2] + 1;
+        const double fadd = p2[jx][jy] * gz[jz] * f[iold];
+        if (finite_d(fadd)) fnew += fadd;
+      }
+    }
+    T[k] = fnew;
+  }
+}
+
+// x part of LagrPol<3> of window node ix of sub-stencil q for a departure point at cx whose
+// window starts at r - 2 (the value interp_weno5 keeps in px[q][ix])
+__device__ __forceinline__ double lag3x(double cx, int r, int q, int ix) {
+  double Fx[3][2];
+  lag3_factors(cx - (r - 2 + q), Fx);
+  return ix == 0 ? 1 * Fx[0][0] * Fx[0][1] : ix == 1 ? 1 * Fx[1][0] * Fx[1][1] : 1 * Fx[2][0] * Fx[2][1];
+}
+
+__device__ __forceinline__ Tgt27 tgt27_of(const Tgt27g3& t, int k) {
+  Tgt27 tk;
+  tk.u1x = t.u1x; tk.u1y = t.u1y; tk.u1z = t.u1z; tk.T1 = t.T1;
+  tk.W = t.W[k]; tk.Cx = t.Cx[k]; tk.Cy = t.Cy; tk.Cz = t.Cz;
+  tk.kx = t.kx[k]; tk.ky = t.ky; tk.kz = t.kz; tk.wok = t.wok[k];
+  return tk;
+}
+
+// interp_weno5<kD3Q27, VAR> for the three targets of a group (cx[k] of target ex = k - 1; cy, cz
+// shared), on a grid that is not a line, without mirrors.  Window positions are taken relative
+// to the cell's own x, so lanes of a wave whose flow is alike visit the same positions.  Taken
+// only for s = sqrt(T1/TLat) <= 1.5 (f64_once): the windows of ex = -1 and +1 are 2s apart, so
+// the union spans at most 9 slices.
+template <int VAR, class SRC>
+__device__ void interp_weno5_g3(const SRC& src, int x, const double (&cx)[3], double cy, double cz,
+                                const Tgt27g3& t, double sqTb, double (&fn)[3]) {
+  constexpr bool CENTRED = VAR == 1;
+  const Grid& g = src.g;
+  const int ry = floor_int(cy + 0.5) - (CENTRED ? 1 : 0), rz = floor_int(cz + 0.5) - (CENTRED ? 1 : 0);
+  const double sy = cy - ry, sz = cz - rz;
+  double Fy[3][2], Fz[3][2];
+  lag3_factors(sy, Fy);
+  lag3_factors(sz, Fz);
+  int r[3], w0[3];  // window start relative to x
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    r[k] = floor_int(cx[k] + 0.5);
+    w0[k] = (r[k] - 2) - x;
+  }
+  const int dlo = min(w0[0], min(w0[1], w0[2]));
+  // (at most dlo + 8 for s <= 1.5; the bound keeps the loop finite whatever the caller passes)
+  const int dhi = min(max(w0[0], max(w0[1], w0[2])) + 4, dlo + 8);
+  double S[3][3], ub[3][5];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+#pragma unroll
+    for (int q = 0; q < 3; q++) S[k][q] = 0;
+#pragma unroll
+    for (int j = 0; j < 5; j++) ub[k][j] = 0;
+  }
+  for (int d = dlo; d <= dhi; d++) {
+    const int xw = wrapi(x + d, g.nx);
+    int xs[3];
+    bool use[3];
+    double pw[3][3];  // per target, the x weight of this slice in sub-stencils 0, 1, 2
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      xs[k] = d - w0[k];
+      use[k] = xs[k] >= 0 && xs[k] <= 4;
+      const int j = xs[k];
+#pragma unroll
+      for (int q = 0; q < 3; q++) pw[k][q] = (j - q >= 0 && j - q <= 2) ? lag3x(cx[k], r[k], q, j - q) : 0.0;
+    }
+#pragma unroll 1
+    for (int a = 0; a < 3; a++) {
+      const int ya = wrapi(ry + a, g.ny);
+#pragma unroll 1
+      for (int b = 0; b < 3; b++) {
+        double T[3] = {0, 0, 0};
+        transfer27_g3(src, node_off(g, xw, ya, wrapi(rz + b, g.nz)), t, use, sqTb, T);
+        const double fy0 = a == 0 ? Fy[0][0] : a == 1 ? Fy[1][0] : Fy[2][0];
+        const double fy1 = a == 0 ? Fy[0][1] : a == 1 ? Fy[1][1] : Fy[2][1];
+        const double fz0 = b == 0 ? Fz[0][0] : b == 1 ? Fz[1][0] : Fz[2][0];
+        const double fz1 = b == 0 ? Fz[0][1] : b == 1 ? Fz[1][1] : Fz[2][1];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          if (!use[k]) continue;
+          const int j = xs[k];
+#pragma unroll
+          for (int q = 0; q < 3; q++)
+            if (j - q >= 0 && j - q <= 2) S[k][q] += pw[k][q] * fy0 * fy1 * fz0 * fz1 * T[k];
+          if (CENTRED && a == 1 && b == 1) {
+#pragma unroll
+            for (int jj = 0; jj < 5; jj++) ub[k][jj] = j == jj ? T[k] : ub[k][jj];
+          }
+        }
+      }
+    }
+    if (!CENTRED) {
+      // smoothness samples on the row (r mod Ny, r mod Nz) of each target
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        if (!use[k]) continue;
+        const double v = transfer27(src, node_off(g, xw, wrapi(r[k], g.ny), wrapi(r[k], g.nz)), tgt27_of(t, k), sqTb);
+#pragma unroll
+        for (int jj = 0; jj < 5; jj++) ub[k][jj] = xs[k] == jj ? v : ub[k][jj];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) fn[k] = weno5_combine_f64<VAR>(cx[k] - r[k], ub[k], S[k][0], S[k][1], S[k][2]);
 }
 
 // propagate<4> (steps.cu:986-1036): tricubic Lagrange on floor(x)-1 .. floor(x)+2.
@@ -594,10 +778,86 @@ struct F64Iter {
   int niter;
 };
 
+// f64_once for D3Q27 WENO5 (INTERP 0, 2, 4) off a line: the targets in (ey, ez) groups of three
+// (interp_weno5_g3), each value stored as it is found; then the moments summed over the stored
+// values in population order, as calcMoments does.
+template <int INTERP, class SRC>
+__device__ __forceinline__ bool f64_once_g3(const SRC& src, const NodeSrc& gsrc, double* __restrict__ fout,
+                                            int64_t cell, int x, int y, int z, bool zero, F64Iter& st,
+                                            const StepArgs& a) {
+  constexpr int VAR = INTERP == 2 ? 1 : INTERP == 4 ? 2 : 0;
+  const int64_t P = a.g.plane;
+  double* c0 = st.c0;
+  st.niter++;
+  const int niter = st.niter;
+  const double uT[4] = {c0[0], c0[1], c0[2], c0[3]};
+  const double s = sqrt(uT[3] / a.tlat);
+  bool conv = true;
+#pragma unroll 1
+  for (int gi = 0; gi < 9; gi++) {
+    const int jy = gi / 3, jz = gi % 3;
+    int idx[3];
+    Tgt27g3 t;
+    double cx[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      idx[k] = kIdx27.v[k][jy][jz];
+      const double vx = s * Lat<kD3Q27>::efc(idx[k], 0) + uT[0];
+      const Tgt27 tk = Lat<kD3Q27>::tgt(idx[k], uT);
+      t.W[k] = tk.W; t.Cx[k] = tk.Cx; t.kx[k] = tk.kx; t.wok[k] = tk.wok;
+      if (k == 0) {
+        t.u1x = tk.u1x; t.u1y = tk.u1y; t.u1z = tk.u1z; t.T1 = tk.T1;
+        t.Cy = tk.Cy; t.Cz = tk.Cz; t.ky = tk.ky; t.kz = tk.kz;
+      }
+      cx[k] = (double)x - vx;
+    }
+    const double vy = s * Lat<kD3Q27>::efc(idx[0], 1) + uT[1];
+    const double vz = s * Lat<kD3Q27>::efc(idx[0], 2) + uT[2];
+    const double cy = (double)y - vy, cz = (double)z - vz;
+    double fn[3];
+    interp_weno5_g3<VAR>(src, x, cx, cy, cz, t, a.sqrt_tbase, fn);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const int64_t o = f64_fidx(kD3Q27, idx[k], cell, P);
+      const double fo = niter > 1 ? fout[o] : (zero ? a.w[idx[k]] * 1.0 : gsrc.f[o]);
+      if (fabs(fn[k] - fo) >= a.tol_abs + a.tol_rel * fabs(fn[k])) conv = false;
+      fout[o] = fn[k];
+    }
+  }
+  double rho = 0, m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+  for (int i = 0; i < 27; i++) {
+    const double vx = s * Lat<kD3Q27>::efc(i, 0) + uT[0];
+    const double vy = s * Lat<kD3Q27>::efc(i, 1) + uT[1];
+    const double vz = s * Lat<kD3Q27>::efc(i, 2) + uT[2];
+    const double fn = fout[f64_fidx(kD3Q27, i, cell, P)];
+    rho += fn;
+    m0 += fn * vx;
+    m1 += fn * vy;
+    m2 += fn * vz;
+    m3 += fn * (vx * vx + vy * vy + vz * vz);
+  }
+  if (rho != 0) { m0 /= rho; m1 /= rho; m2 /= rho; m3 /= rho; }
+  m3 = (m3 - m0 * m0 - m1 * m1 - m2 * m2) * a.inv_dim;
+  if (m3 < 0) m3 = c0[3];
+  const double mn[4] = {m0, m1, m2, m3};
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (fabs(mn[k] - c0[k]) >= a.tol_abs + a.tol_rel * fabs(mn[k])) conv = false;
+  st.rho0 = rho;
+  c0[0] = m0; c0[1] = m1; c0[2] = m2; c0[3] = m3;
+  return conv;
+}
+
 // One fixed-point iteration (LBMpull, steps.cu:421-512) of one cell: true when it has converged.
 template <int LAT, int INTERP, class SRC>
 __device__ __forceinline__ bool f64_once(const SRC& src, const NodeSrc& gsrc, double* __restrict__ fout, int64_t cell,
                                          int x, int y, int z, bool zero, F64Iter& st, const StepArgs& a) {
+  if constexpr (LAT == kD3Q27 && (INTERP == 0 || INTERP == 2 || INTERP == 4)) {
+    // the grouped path off a line, for gauges whose targets' windows lie close (s <= 1.5, not
+    // NaN); hotter or non-finite ones, one target at a time below
+    if ((a.g.ny > 1 || a.g.nz > 1) && sqrt(st.c0[3] / a.tlat) <= 1.5)
+      return f64_once_g3<INTERP>(src, gsrc, fout, cell, x, y, z, zero, st, a);
+  }
   constexpr int Q = Lat<LAT>::Q;
   const int64_t P = a.g.plane;
   double* c0 = st.c0;
@@ -694,8 +954,12 @@ __device__ __forceinline__ void f64_finish(double* __restrict__ fout, double* __
 // cells on shocks and contact discontinuities needs several times the mean count.  Each cell's
 // arithmetic and operation order are those of a single pass.
 // BUF: node loads through buffer resources (planes up to 59,652,323 cells), else global loads.
+// D3Q27: held to two waves per SIMD (256 VGPRs, a few spilled).  Measured on the 384^3
+// shock-bubble FP64 step, A/B on one GPU: the grouped WENO5C kernel 2.70 s at one wave per SIMD
+// (its registers spilled to AGPRs), 2.58 s at two; Lagrange-4 5.34 s -> 4.99 s.  The D1Q5 kernel
+// keeps its registers (held to two waves it would spill 1.7 KB per lane).
 template <int LAT, int INTERP, bool BUF>
-__global__ __launch_bounds__(64) void k_step_f64(const NodeSrc gsrc, double* __restrict__ fout,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LAT == kD3Q27 ? 2 : 1))) void k_step_f64(const NodeSrc gsrc, double* __restrict__ fout,
                                                  double* __restrict__ mout, int* __restrict__ niter_out, Pending pend,
                                                  const StepArgs a) {
   using Src = typename std::conditional<BUF, NodeBuf, NodeSrc>::type;
@@ -733,7 +997,7 @@ __global__ __launch_bounds__(64) void k_step_f64(const NodeSrc gsrc, double* __r
 // a.pass1_iters iterations, to the pending list), instead of idling until the slowest cell of
 // their tile is done.  Each cell's arithmetic is that of k_step_f64.
 template <int LAT, int INTERP, bool BUF>
-__global__ __launch_bounds__(64) void k_step_f64_refill(const NodeSrc gsrc, double* __restrict__ fout,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LAT == kD3Q27 ? 2 : 1))) void k_step_f64_refill(const NodeSrc gsrc, double* __restrict__ fout,
                                                         double* __restrict__ mout, int* __restrict__ niter_out,
                                                         Pending pend, const StepArgs a, int chunk) {
   using Src = typename std::conditional<BUF, NodeBuf, NodeSrc>::type;
@@ -791,7 +1055,7 @@ __global__ __launch_bounds__(64) void k_step_f64_refill(const NodeSrc gsrc, doub
 
 // Second pass: a fixed grid of waves walks the pending list, every cell resumes its iteration
 template <int LAT, int INTERP, bool BUF>
-__global__ __launch_bounds__(64) void k_step_f64_rest(const NodeSrc gsrc, double* __restrict__ fout,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LAT == kD3Q27 ? 2 : 1))) void k_step_f64_rest(const NodeSrc gsrc, double* __restrict__ fout,
                                                       double* __restrict__ mout, int* __restrict__ niter_out,
                                                       Pending pend, const StepArgs a) {
   using Src = typename std::conditional<BUF, NodeBuf, NodeSrc>::type;
