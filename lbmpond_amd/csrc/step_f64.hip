@@ -713,6 +713,70 @@ __device__ void interp_weno5_g3(const SRC& src, int x, const double (&cx)[3], do
   for (int k = 0; k < 3; k++) fn[k] = weno5_combine_f64<VAR>(cx[k] - r[k], ub[k], S[k][0], S[k][1], S[k][2]);
 }
 
+// x part of LagrPol<4> of node ix for a departure point at cx whose window starts at lx (the
+// value interp_lagr4 forms as p)
+__device__ __forceinline__ double lag4x(double cx, int lx, int ix) {
+  double Fx[4][3];
+  lag4_factors(cx - lx, Fx);
+  return ix == 0   ? 1 * Fx[0][0] * Fx[0][1] * Fx[0][2]
+         : ix == 1 ? 1 * Fx[1][0] * Fx[1][1] * Fx[1][2]
+         : ix == 2 ? 1 * Fx[2][0] * Fx[2][1] * Fx[2][2]
+                   : 1 * Fx[3][0] * Fx[3][1] * Fx[3][2];
+}
+
+// interp_lagr4<kD3Q27, false> for the three targets of a group, as interp_weno5_g3 does for
+// WENO5: each node of the union of their windows (at most 8 slices for s <= 1.5) loaded once,
+// each target's sum in its own node order.
+template <class SRC>
+__device__ void interp_lagr4_g3(const SRC& src, int x, const double (&cx)[3], double cy, double cz,
+                                const Tgt27g3& t, double sqTb, double (&fn)[3]) {
+  const Grid& g = src.g;
+  const int ly = floor_int(cy) - 4 / 2 + 1;
+  const int lz = floor_int(cz) - 4 / 2 + 1;
+  double Fy[4][3], Fz[4][3];
+  lag4_factors(cy - ly, Fy);
+  lag4_factors(cz - lz, Fz);
+  int lx[3], w0[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    lx[k] = floor_int(cx[k]) - 4 / 2 + 1;
+    w0[k] = lx[k] - x;
+    fn[k] = 0;
+  }
+  const int dlo = min(w0[0], min(w0[1], w0[2]));
+  // (at most dlo + 7 for s <= 1.5; the bound keeps the loop finite whatever the caller passes)
+  const int dhi = min(max(w0[0], max(w0[1], w0[2])) + 3, dlo + 7);
+  for (int d = dlo; d <= dhi; d++) {
+    const int xw = wrapi(x + d, g.nx);
+    bool use[3];
+    double p[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const int ix = d - w0[k];
+      use[k] = ix >= 0 && ix <= 3;
+      p[k] = use[k] ? lag4x(cx[k], lx[k], ix) : 0.0;
+    }
+#pragma unroll 1
+    for (int a = 0; a < 4; a++) {
+      const int ya = wrapi(ly + a, g.ny);
+      const double fy0 = a == 0 ? Fy[0][0] : a == 1 ? Fy[1][0] : a == 2 ? Fy[2][0] : Fy[3][0];
+      const double fy1 = a == 0 ? Fy[0][1] : a == 1 ? Fy[1][1] : a == 2 ? Fy[2][1] : Fy[3][1];
+      const double fy2 = a == 0 ? Fy[0][2] : a == 1 ? Fy[1][2] : a == 2 ? Fy[2][2] : Fy[3][2];
+#pragma unroll 1
+      for (int b = 0; b < 4; b++) {
+        double T[3] = {0, 0, 0};
+        transfer27_g3(src, node_off(g, xw, ya, wrapi(lz + b, g.nz)), t, use, sqTb, T);
+        const double fz0 = b == 0 ? Fz[0][0] : b == 1 ? Fz[1][0] : b == 2 ? Fz[2][0] : Fz[3][0];
+        const double fz1 = b == 0 ? Fz[0][1] : b == 1 ? Fz[1][1] : b == 2 ? Fz[2][1] : Fz[3][1];
+        const double fz2 = b == 0 ? Fz[0][2] : b == 1 ? Fz[1][2] : b == 2 ? Fz[2][2] : Fz[3][2];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+          if (use[k]) fn[k] += p[k] * fy0 * fy1 * fy2 * fz0 * fz1 * fz2 * T[k];
+      }
+    }
+  }
+}
+
 // propagate<4> (steps.cu:986-1036): tricubic Lagrange on floor(x)-1 .. floor(x)+2.
 // MIR: mirror boundaries on the axes of g.mirror (steps.cu:1004-1022): nodes beyond a mirrored
 // face are the mirror images of the cells inside.
@@ -778,9 +842,10 @@ struct F64Iter {
   int niter;
 };
 
-// f64_once for D3Q27 WENO5 (INTERP 0, 2, 4) off a line: the targets in (ey, ez) groups of three
-// (interp_weno5_g3), each value stored as it is found; then the moments summed over the stored
-// values in population order, as calcMoments does.
+// f64_once for D3Q27 WENO5 (INTERP 0, 2, 4) and Lagrange-4 (1) off a line, without mirrors: the
+// targets in (ey, ez) groups of three (interp_weno5_g3, interp_lagr4_g3), each value stored as it
+// is found; then the moments summed over the stored values in population order, as calcMoments
+// does.
 template <int INTERP, class SRC>
 __device__ __forceinline__ bool f64_once_g3(const SRC& src, const NodeSrc& gsrc, double* __restrict__ fout,
                                             int64_t cell, int x, int y, int z, bool zero, F64Iter& st,
@@ -815,7 +880,8 @@ __device__ __forceinline__ bool f64_once_g3(const SRC& src, const NodeSrc& gsrc,
     const double vz = s * Lat<kD3Q27>::efc(idx[0], 2) + uT[2];
     const double cy = (double)y - vy, cz = (double)z - vz;
     double fn[3];
-    interp_weno5_g3<VAR>(src, x, cx, cy, cz, t, a.sqrt_tbase, fn);
+    if constexpr (INTERP == 1) interp_lagr4_g3(src, x, cx, cy, cz, t, a.sqrt_tbase, fn);
+    else interp_weno5_g3<VAR>(src, x, cx, cy, cz, t, a.sqrt_tbase, fn);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       const int64_t o = f64_fidx(kD3Q27, idx[k], cell, P);
@@ -852,7 +918,7 @@ __device__ __forceinline__ bool f64_once_g3(const SRC& src, const NodeSrc& gsrc,
 template <int LAT, int INTERP, class SRC>
 __device__ __forceinline__ bool f64_once(const SRC& src, const NodeSrc& gsrc, double* __restrict__ fout, int64_t cell,
                                          int x, int y, int z, bool zero, F64Iter& st, const StepArgs& a) {
-  if constexpr (LAT == kD3Q27 && (INTERP == 0 || INTERP == 2 || INTERP == 4)) {
+  if constexpr (LAT == kD3Q27 && (INTERP == 0 || INTERP == 1 || INTERP == 2 || INTERP == 4)) {
     // the grouped path off a line, for gauges whose targets' windows lie close (s <= 1.5, not
     // NaN); hotter or non-finite ones, one target at a time below
     if ((a.g.ny > 1 || a.g.nz > 1) && sqrt(st.c0[3] / a.tlat) <= 1.5)
