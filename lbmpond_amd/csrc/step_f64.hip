@@ -652,13 +652,17 @@ __device__ void interp_weno5_g3(const SRC& src, int x, const double (&cx)[3], do
   const int dlo = min(w0[0], min(w0[1], w0[2]));
   // (at most dlo + 8 for s <= 1.5; the bound keeps the loop finite whatever the caller passes)
   const int dhi = min(max(w0[0], max(w0[1], w0[2])) + 4, dlo + 8);
-  double S[3][3], ub[3][5];
+  // the smoothness samples of the three targets wait in LDS (one wave per block), which keeps
+  // the sub-stencil sums and a node's populations in registers
+  __shared__ double s_ub[3][5][64];
+  const int lane = threadIdx.x & 63;
+  double S[3][3];
 #pragma unroll
   for (int k = 0; k < 3; k++) {
 #pragma unroll
     for (int q = 0; q < 3; q++) S[k][q] = 0;
 #pragma unroll
-    for (int j = 0; j < 5; j++) ub[k][j] = 0;
+    for (int j = 0; j < 5; j++) s_ub[k][j][lane] = 0;
   }
   for (int d = dlo; d <= dhi; d++) {
     const int xw = wrapi(x + d, g.nx);
@@ -691,10 +695,7 @@ __device__ void interp_weno5_g3(const SRC& src, int x, const double (&cx)[3], do
 #pragma unroll
           for (int q = 0; q < 3; q++)
             if (j - q >= 0 && j - q <= 2) S[k][q] += pw[k][q] * fy0 * fy1 * fz0 * fz1 * T[k];
-          if (CENTRED && a == 1 && b == 1) {
-#pragma unroll
-            for (int jj = 0; jj < 5; jj++) ub[k][jj] = j == jj ? T[k] : ub[k][jj];
-          }
+          if (CENTRED && a == 1 && b == 1) s_ub[k][j][lane] = T[k];
         }
       }
     }
@@ -703,14 +704,18 @@ __device__ void interp_weno5_g3(const SRC& src, int x, const double (&cx)[3], do
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         if (!use[k]) continue;
-        const double v = transfer27(src, node_off(g, xw, wrapi(r[k], g.ny), wrapi(r[k], g.nz)), tgt27_of(t, k), sqTb);
-#pragma unroll
-        for (int jj = 0; jj < 5; jj++) ub[k][jj] = xs[k] == jj ? v : ub[k][jj];
+        s_ub[k][xs[k]][lane] =
+            transfer27(src, node_off(g, xw, wrapi(r[k], g.ny), wrapi(r[k], g.nz)), tgt27_of(t, k), sqTb);
       }
     }
   }
 #pragma unroll
-  for (int k = 0; k < 3; k++) fn[k] = weno5_combine_f64<VAR>(cx[k] - r[k], ub[k], S[k][0], S[k][1], S[k][2]);
+  for (int k = 0; k < 3; k++) {
+    double ub[5];
+#pragma unroll
+    for (int j = 0; j < 5; j++) ub[j] = s_ub[k][j][lane];
+    fn[k] = weno5_combine_f64<VAR>(cx[k] - r[k], ub, S[k][0], S[k][1], S[k][2]);
+  }
 }
 
 // x part of LagrPol<4> of node ix for a departure point at cx whose window starts at lx (the
