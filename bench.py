@@ -344,7 +344,11 @@ def main():
                 ref = run.p.fields()
                 entry = {"from_step": pending["from"], "fp64_capped_cells": int((ref["niter"] >= 100).sum())}
                 for name, got in pending["fields"].items():
+                    # where either step stopped at the cap its value is an unconverged iterate
+                    # (of a cycle, in the FP64 step too), not a fixed point to compare
+                    conv = (got["niter"] < 100) & (ref["niter"] < 100)
                     entry[name] = {"all_cells": errs(got, ref),
+                                   "converged_cells": errs(got, ref, conv) if not conv.all() else errs(got, ref),
                                    "capped_cells_here_not_fp64": int(((got["niter"] >= 100) & (ref["niter"] < 100)).sum()),
                                    "niter_mean": round(float(got["niter"].mean()), 4)}
                 accuracy["one_step"].append(entry)
@@ -411,10 +415,13 @@ def main():
     one = accuracy["one_step"]
     if one:
         worst = {k: max(e[head_name]["all_cells"][k] for e in one) for k in ("rho", "u", "T")}
+        worst_conv = {k: max(e[head_name]["converged_cells"][k] for e in one) for k in ("rho", "u", "T")}
         accuracy["gate_rel_linf"] = worst
         accuracy["gate_definition"] = ("max over the one-step checks of the relative L-inf error of the headline "
                                        "path against the FP64 step from the same FP64 state, all cells")
         accuracy["gate_met"] = all(v <= ACCURACY_GATE for v in worst.values())
+        accuracy["gate_rel_linf_converged_cells"] = worst_conv
+        accuracy["gate_met_converged_cells"] = all(v <= ACCURACY_GATE for v in worst_conv.values())
     if ref is not None and head is not ref:
         accuracy["trajectory"] = summary[head_name]["trajectory_rel_linf_vs_fp64"]
     max_dep = max(r.res["max_departure"] for r in results.values())
