@@ -25,18 +25,22 @@ constexpr int kLearnedPrevMin = 3, kLearnedPrevMax = 4;
 
 // D3Q27 split populations are stored in seven quad planes: population k of cell c at float index
 // ((k / 4) * plane + c) * 4 + k % 4 (16 bytes per cell and quad plane, the 28th slot unused), so
-// a stencil node's 27 populations are seven 16-byte loads.  D3Q125 keeps one plane per
-// population (index k * plane + c).
-constexpr int kSplit27Floats = 28;  // floats per cell of the D3Q27 split population storage
+// a stencil node's 27 populations are seven 16-byte loads.  D3Q125 stores each cell's 125
+// populations as one 512-byte record (index 128 c + k, three floats unused): the step's block for
+// a cell reads each of its 27 neighbours' populations as one contiguous record (one population
+// per plane put every read in its own cache line and page: at 256^3 the step was bound by that
+// gather).
+constexpr int kSplit27Floats = 28;   // floats per cell of the D3Q27 split population storage
+constexpr int kSplit125Floats = 128;  // floats per cell of the D3Q125 split population storage
 __host__ __device__ inline int64_t split27_fidx(int k, int64_t c, int64_t plane) {
   return (((int64_t)(k >> 2) * plane + c) << 2) + (k & 3);
 }
 __host__ __device__ inline int64_t split_fidx(int lattice, int k, int64_t c, int64_t plane) {
-  return lattice == kD3Q27 ? split27_fidx(k, c, plane) : (int64_t)k * plane + c;
+  return lattice == kD3Q27 ? split27_fidx(k, c, plane) : c * kSplit125Floats + k;
 }
 // floats of one split population buffer
 inline int64_t split_fcount(int lattice, int qn, int64_t plane) {
-  return (lattice == kD3Q27 ? kSplit27Floats : qn) * plane;
+  return (lattice == kD3Q27 ? kSplit27Floats : kSplit125Floats) * plane;
 }
 
 struct SplitPtrs {

@@ -1317,6 +1317,8 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
   __shared__ float su[27][3];   // neighbourhood gauge velocities
   __shared__ float ss0[27];     // neighbourhood sqrt(|T|)*sign(T)
   __shared__ double red[10];    // per-wave partial sums of the five moments
+  __shared__ float sL[2][3][5][5];  // a node's basis values L[d][target index][source index], by node parity
+  __shared__ float sTz[Q], sTy[Q];  // a node's z sums and y sums
   const Grid& g = a.g;
   const int64_t P = g.plane;
   const int64_t cell = xcd_block(blockIdx.x, gridDim.x);
@@ -1345,7 +1347,7 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
     const float s0 = (float)(sqrt(fabs(t)) * (double)((0.0 < t) - (t < 0.0)));
     const bool gok = isfinite(ux) && isfinite(uy) && isfinite(uz) && isfinite(s0);
     if (lane_ok) {
-      const float v = fin[(int64_t)kl * P + c];
+      const float v = fin[c * kSplit125Floats + kl];
       sf[n][k] = gok && isfinite(v) ? v : 0.0f;
     }
     if (k < 3) su[n][k] = gok ? (k == 0 ? ux : k == 1 ? uy : uz) : 0.0f;
@@ -1383,20 +1385,6 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
     // target basis constants (structs.cuh:397-416)
     const double S1 = sqrt(fabs(gc[3])) * (double)((0.0 < gc[3]) - (gc[3] < 0.0));
     const double e1 = kEc1 * S1, e2 = kEc2 * S1;
-    float wv[3];
-    float nd[3][4];  // the other four target nodes of axis d
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      const double xn = node5(X[d], e1, e2);
-      double den = 1.0;
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const double v = node5(q < X[d] ? q : q + 1, e1, e2);
-        den *= xn - v;
-        nd[d][q] = (float)v;
-      }
-      wv[d] = (float)(1.0 / den);
-    }
     // departure point on the fixed stencil coord-1..coord+1 (steps.cu:163,207-208)
     float Fx[3], Fy[3], Fz[3];
     {
@@ -1407,43 +1395,65 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
     }
     const float u1[3] = {(float)gc[0], (float)gc[1], (float)gc[2]};
     const float u1l[3] = {(float)(gc[0] - (double)u1[0]), (float)(gc[1] - (double)u1[1]), (float)(gc[2] - (double)u1[2])};
-    float acc = 0.f;
-    for (int ia = 0; ia < 3; ia++) {
-      float accy = 0.f;
-      for (int ib = 0; ib < 3; ib++) {
-        float accz = 0.f;
-#pragma unroll 1
-        for (int ic = 0; ic < 3; ic++) {
-          const int n = 9 * ia + 3 * ib + ic;
-          const float S0 = ss0[n];
-          float L[3][5];
+    // Transfer of each node to every target, contracted one axis at a time across the block:
+    // the basis values L[d][kd][j] of the node (75, one per lane), the z sums tz(jx, jy, kz), the
+    // y sums ty(jx, ky, kz), then each target's x sum, every value formed once and read by the
+    // targets that share it.  A target's value is the same sequence of operations as contracting
+    // its own 125 terms (z inner, then y, then x).
+    const int tb = lane_ok ? k : 0;
+    const int bd = tb / 25, bk = (tb / 5) % 5, bj = tb % 5;  // basis entry of lanes 0..74
+    float bw = 0.f, bn[4] = {0.f, 0.f, 0.f, 0.f};
+    {
+      const double xn = node5(bk, e1, e2);
+      double den = 1.0;
 #pragma unroll
-          for (int d = 0; d < 3; d++) {
-            const float du = (su[n][d] - u1[d]) - u1l[d];
-#pragma unroll
-            for (int j = 0; j < 5; j++) {
-              const float b = du + (float)kA5d[j] * S0;
-              L[d][j] = wv[d] * ((b - nd[d][0]) * (b - nd[d][1]) * (b - nd[d][2]) * (b - nd[d][3]));
-            }
-          }
-          float tx = 0.f;
-#pragma unroll
-          for (int jx = 0; jx < 5; jx++) {
-            float ty = 0.f;
-#pragma unroll
-            for (int jy = 0; jy < 5; jy++) {
-              float tz = 0.f;
-#pragma unroll
-              for (int jz = 0; jz < 5; jz++) tz = __builtin_fmaf(L[2][jz], sf[n][25 * jx + 5 * jy + jz], tz);
-              ty = __builtin_fmaf(L[1][jy], tz, ty);
-            }
-            tx = __builtin_fmaf(L[0][jx], ty, tx);
-          }
-          accz = __builtin_fmaf(Fz[ic], tx, accz);
-        }
-        accy = __builtin_fmaf(Fy[ib], accz, accy);
+      for (int q = 0; q < 4; q++) {
+        const double v = node5(q < bk ? q : q + 1, e1, e2);
+        den *= xn - v;
+        bn[q] = (float)v;
       }
-      acc = __builtin_fmaf(Fx[ia], accy, acc);
+      bw = (float)(1.0 / den);
+    }
+    float acc = 0.f, accy = 0.f, accz = 0.f;
+#pragma unroll 1
+    for (int n = 0; n < 27; n++) {
+      const int pb = n & 1;
+      if (k < 75) {
+        const float du = (su[n][bd] - u1[bd]) - u1l[bd];
+        const float bv = du + (float)kA5d[bj] * ss0[n];
+        sL[pb][bd][bk][bj] = bw * ((bv - bn[0]) * (bv - bn[1]) * (bv - bn[2]) * (bv - bn[3]));
+      }
+      __syncthreads();
+      if (lane_ok) {  // tz(jx, jy, kz) at index 25 jx + 5 jy + kz
+        float tz = 0.f;
+#pragma unroll
+        for (int jz = 0; jz < 5; jz++) tz = __builtin_fmaf(sL[pb][2][bj][jz], sf[n][25 * bd + 5 * bk + jz], tz);
+        sTz[k] = tz;
+      }
+      __syncthreads();
+      if (lane_ok) {  // ty(jx, ky, kz) at index 25 jx + 5 ky + kz
+        float ty = 0.f;
+#pragma unroll
+        for (int jy = 0; jy < 5; jy++) ty = __builtin_fmaf(sL[pb][1][bk][jy], sTz[25 * bd + 5 * jy + bj], ty);
+        sTy[k] = ty;
+      }
+      __syncthreads();
+      float tx = 0.f;
+      if (lane_ok) {
+#pragma unroll
+        for (int jx = 0; jx < 5; jx++) tx = __builtin_fmaf(sL[pb][0][X[0]][jx], sTy[25 * jx + 5 * X[1] + X[2]], tx);
+      }
+      // (the stencil sums of the fixed 3x3x3 neighbourhood: z inner, then y, then x)
+      const int ia = n / 9, ib = (n / 3) % 3, ic = n % 3;
+      accz = __builtin_fmaf(Fz[ic], tx, accz);
+      if (ic == 2) {
+        accy = __builtin_fmaf(Fy[ib], accz, accy);
+        accz = 0.f;
+        if (ib == 2) {
+          acc = __builtin_fmaf(Fx[ia], accy, acc);
+          accy = 0.f;
+        }
+      }
     }
     fn = lane_ok ? acc : 0.f;
     // moments in the frame of the current gauge, FP64 block sums
@@ -1498,7 +1508,7 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
     for (int d = 0; d < 4; d++) gc[d] = m[d];
   }
   const float rf = (float)rho;
-  if (lane_ok) fout[(int64_t)k * P + cell] = __builtin_fmaf((float)a.dtau, rf * a.w[k] - fn, fn);
+  if (lane_ok) fout[cell * kSplit125Floats + k] = __builtin_fmaf((float)a.dtau, rf * a.w[k] - fn, fn);
   if (k == 0) {
     mout[cell] = rf;
     mout[P + cell] = (float)gc[0];
