@@ -145,12 +145,19 @@ __device__ __forceinline__ bool tile_cell(const Grid& g, int64_t b, int lane, in
 // FP64 population storage.  D3Q27 keeps the populations in fourteen pair planes: population k
 // of cell c at double index ((k / 2) * plane + c) * 2 + k % 2 (16 bytes per cell and pair
 // plane, the 28th slot unused), so a stencil node's 27 populations are fourteen 16-byte loads.
-// D1Q5 and D3Q125 keep one plane per population (index k * plane + c).
+// D1Q5 keeps one plane per population (index k * plane + c).  D3Q125 keeps each cell's 125
+// populations as one 1 KB record (index 128 c + k, three doubles unused): its step's block for a
+// cell reads the whole records of the cell's 27 neighbours.
+constexpr int kF64Q125Doubles = 128;
 __host__ __device__ inline int64_t f64_fidx(int lattice, int k, int64_t c, int64_t plane) {
-  return lattice == kD3Q27 ? ((((int64_t)(k >> 1) * plane + c) << 1) + (k & 1)) : (int64_t)k * plane + c;
+  return lattice == kD3Q27    ? ((((int64_t)(k >> 1) * plane + c) << 1) + (k & 1))
+         : lattice == kD3Q125 ? c * kF64Q125Doubles + k
+                              : (int64_t)k * plane + c;
 }
 // doubles of one FP64 population buffer
-inline int64_t f64_fcount(int lattice, int qn, int64_t plane) { return (lattice == kD3Q27 ? 28 : qn) * plane; }
+inline int64_t f64_fcount(int lattice, int qn, int64_t plane) {
+  return (lattice == kD3Q27 ? 28 : lattice == kD3Q125 ? kF64Q125Doubles : qn) * plane;
+}
 
 __device__ __forceinline__ int64_t cell_index(const Grid& g, int x, int y, int z) {
   return (int64_t)x + (int64_t)g.nx * ((int64_t)y + (int64_t)g.ny * (int64_t)z);

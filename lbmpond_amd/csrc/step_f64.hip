@@ -1183,7 +1183,7 @@ __global__ __launch_bounds__(128) void k_step_q125_f64(const double* __restrict_
     const int kq = lane_ok ? k : 0;  // lanes 125..127 hold no population
     const int kx = kq / 25, ky = (kq / 5) % 5, kz = kq % 5;
     const int kl = 25 * (rx ? kMir5[kx] : kx) + 5 * (ry ? kMir5[ky] : ky) + (rz ? kMir5[kz] : kz);
-    if (lane_ok) sf[n][k] = fin[(int64_t)kl * P + c];
+    if (lane_ok) sf[n][k] = fin[f64_fidx(kD3Q125, kl, c, P)];
     if (k < 4) {
       const double v = min[(int64_t)(k + 1) * P + c];
       sg[n][k] = (k == 0 && rx) || (k == 1 && ry) || (k == 2 && rz) ? -v : v;
@@ -1193,7 +1193,7 @@ __global__ __launch_bounds__(128) void k_step_q125_f64(const double* __restrict_
   double rho0 = min[cell], c0[4] = {min[P + cell], min[2 * P + cell], min[3 * P + cell], min[4 * P + cell]};
   const bool zero = rho0 == 0 && c0[0] == 0 && c0[1] == 0 && c0[2] == 0 && c0[3] == 0;
   if (zero) { rho0 = 1; c0[3] = a.tlat; }
-  double f0 = lane_ok ? (zero ? a.w[k] * 1.0 : fin[(int64_t)k * P + cell]) : 0.0;
+  double f0 = lane_ok ? (zero ? a.w[k] * 1.0 : fin[f64_fidx(kD3Q125, k, cell, P)]) : 0.0;
   __syncthreads();
 
   // per-axis class of target k: index into {0,+ec1,-ec1,+ec2,-ec2}
@@ -1314,7 +1314,7 @@ __global__ __launch_bounds__(128) void k_step_q125_f64(const double* __restrict_
   if (lane_ok) {
     const double feq = rho0 * a.w[k];
     f0 += a.dtau * (feq - f0);
-    fout[(int64_t)k * P + cell] = f0;
+    fout[f64_fidx(kD3Q125, k, cell, P)] = f0;
   }
   if (k == 0) {
     mout[cell] = rho0;
