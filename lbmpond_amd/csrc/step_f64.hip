@@ -1020,10 +1020,10 @@ __device__ __forceinline__ void f64_finish(double* __restrict__ fout, double* __
 }
 
 // First pass: every cell iterates up to a.pass1_iters times; cells still unconverged are
-// appended to the pending list (one atomic per wave) and finished by k_step_f64_rest, whose
-// waves hold only such cells.  A wave runs as long as its slowest cell, and a thin layer of
-// cells on shocks and contact discontinuities needs several times the mean count.  Each cell's
-// arithmetic and operation order are those of a single pass.
+// appended to the pending list (one atomic per wave) and finished by a second pass over that
+// list (k_step_f64_rest or k_step_f64_rest_wide).  A wave runs as long as its slowest cell, and
+// a thin layer of cells on shocks and contact discontinuities needs several times the mean
+// count.  Each cell's arithmetic and operation order are those of a single pass.
 // BUF: node loads through buffer resources (planes up to 59,652,323 cells), else global loads.
 // D3Q27: held to two waves per SIMD (256 VGPRs, a few spilled).  Measured on the 384^3
 // shock-bubble FP64 step, A/B on one GPU: the grouped WENO5C kernel 2.70 s at one wave per SIMD
@@ -1148,6 +1148,103 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LAT == kD3Q2
     // the zero-cell start only matters for the first iteration, which the first pass ran
     f64_iterate<LAT, INTERP>(src, gsrc, fout, cell, x, y, z, false, st, a.max_iter, a);
     f64_finish<Lat<LAT>::Q>(fout, mout, niter_out, cell, st, a);
+  }
+}
+
+// Second pass, one wave per unconverged cell and one lane per target population: each lane
+// interpolates its own population with the one-target code, the wave then sums the moments over
+// the populations in order from LDS (every lane the same sums), so every value is the one
+// f64_once gives, while an iteration of a slow cell costs one population's interpolation instead
+// of all of them in one lane.  The waves walk the pending list.  Used for the D3Q27 WENO5 steps
+// (384^3 shock-bubble FP64 WENO5C step 2079 -> 1948 ms); the Lagrange-4 step keeps the one-lane
+// pass, whose cells take the grouped path (2587 ms against 2712 wide).
+template <int LAT, int INTERP, bool BUF>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LAT == kD3Q27 ? 2 : 1))) void k_step_f64_rest_wide(const NodeSrc gsrc, double* __restrict__ fout,
+                                                           double* __restrict__ mout, int* __restrict__ niter_out,
+                                                           Pending pend, const StepArgs a) {
+  using Src = typename std::conditional<BUF, NodeBuf, NodeSrc>::type;
+  Src src;
+  if constexpr (BUF) src = make_nodebuf<LAT>(gsrc);
+  else src = gsrc;
+  constexpr int Q = Lat<LAT>::Q;
+  constexpr int VAR = INTERP == 2 ? 1 : INTERP == 4 ? 2 : 0;
+  __shared__ double sfn[Q];
+  const Grid& g = a.g;
+  const int64_t P = g.plane;
+  const int i = threadIdx.x;
+  const bool lane_ok = i < Q;
+  const int count = *pend.count;
+  for (int64_t e = blockIdx.x; e < count; e += gridDim.x) {
+    const int64_t cell = pend.cell[e];
+    const int x = (int)(cell % g.nx);
+    const int y = (int)((cell / g.nx) % g.ny);
+    const int z = (int)(cell / ((int64_t)g.nx * g.ny));
+    double c0[4];
+    for (int k = 0; k < 4; k++) c0[k] = pend.g[k * pend.cap + e];
+    double rho0 = pend.res[e];
+    int niter = pend.meta[e];
+    const int64_t o = lane_ok ? f64_fidx(LAT, i, cell, P) : 0;
+    while (niter < a.max_iter) {
+      niter++;
+      const double uT[4] = {c0[0], c0[1], c0[2], c0[3]};
+      const double s = sqrt(uT[3] / a.tlat);
+      bool conv_i = true;
+      if (lane_ok) {
+        const double vx = s * Lat<LAT>::efc(i, 0) + uT[0];
+        const double vy = s * Lat<LAT>::efc(i, 1) + uT[1];
+        const double vz = s * Lat<LAT>::efc(i, 2) + uT[2];
+        const typename Lat<LAT>::Tgt t = Lat<LAT>::tgt(i, uT);
+        const double cx = (double)x - vx, cy = (double)y - vy, cz = (double)z - vz;
+        const double fn = INTERP == 1   ? interp_lagr4<LAT, false>(src, cx, cy, cz, t, a.sqrt_tbase)
+                          : INTERP == 3 ? interp_lagr4<LAT, true>(src, cx, cy, cz, t, a.sqrt_tbase)
+                                        : interp_weno5<LAT, VAR>(src, cx, cy, cz, t, a.sqrt_tbase);
+        const double fo = fout[o];  // the previous iterate (the first pass ran at least one)
+        if (fabs(fn - fo) >= a.tol_abs + a.tol_rel * fabs(fn)) conv_i = false;
+        fout[o] = fn;
+        sfn[i] = fn;
+      }
+      __syncthreads();
+      // Cell::calcMoments (structs.cuh:329-330), population order, in every lane
+      double rho = 0, m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+      for (int j = 0; j < Q; j++) {
+        const double vx = s * Lat<LAT>::efc(j, 0) + uT[0];
+        const double vy = s * Lat<LAT>::efc(j, 1) + uT[1];
+        const double vz = s * Lat<LAT>::efc(j, 2) + uT[2];
+        const double fn = sfn[j];
+        rho += fn;
+        m0 += fn * vx;
+        m1 += fn * vy;
+        m2 += fn * vz;
+        m3 += fn * (vx * vx + vy * vy + vz * vz);
+      }
+      __syncthreads();  // sfn is rewritten next iteration
+      if (rho != 0) { m0 /= rho; m1 /= rho; m2 /= rho; m3 /= rho; }
+      m3 = (m3 - m0 * m0 - m1 * m1 - m2 * m2) * a.inv_dim;
+      if (m3 < 0) m3 = c0[3];
+      const double mn[4] = {m0, m1, m2, m3};
+      bool conv = __ballot(!conv_i) == 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if (fabs(mn[k] - c0[k]) >= a.tol_abs + a.tol_rel * fabs(mn[k])) conv = false;
+      rho0 = rho;
+      c0[0] = m0; c0[1] = m1; c0[2] = m2; c0[3] = m3;
+      if (conv) break;
+    }
+    // f64_finish, one population per lane
+    if (lane_ok) {
+      double f = fout[o];
+      const double feq = rho0 * a.w[i];
+      f += a.dtau * (feq - f);
+      fout[o] = f;
+    }
+    if (i == 0) {
+      mout[cell] = rho0;
+      mout[P + cell] = c0[0];
+      mout[2 * P + cell] = c0[1];
+      mout[3 * P + cell] = c0[2];
+      mout[4 * P + cell] = c0[3];
+      niter_out[cell] = niter;
+    }
   }
 }
 
@@ -1362,6 +1459,18 @@ hipError_t launch_node_prep(int lattice, const double* f, const double* m, doubl
   return hipGetLastError();
 }
 
+// the second pass: wide for the D3Q27 WENO5 steps, one lane per cell otherwise
+template <int LAT, int INTERP, bool BUF>
+static void launch_rest(unsigned blocks2, hipStream_t st, const NodeSrc& src, double* fout, double* mout, int* niter,
+                        const Pending& pend, const StepArgs& a) {
+  if constexpr (LAT == kD3Q27 && (INTERP == 0 || INTERP == 2 || INTERP == 4))
+    hipLaunchKernelGGL((k_step_f64_rest_wide<LAT, INTERP, BUF>), dim3(blocks2), dim3(64), 0, st, src, fout, mout, niter,
+                       pend, a);
+  else
+    hipLaunchKernelGGL((k_step_f64_rest<LAT, INTERP, BUF>), dim3(blocks2), dim3(64), 0, st, src, fout, mout, niter,
+                       pend, a);
+}
+
 hipError_t launch_step_f64(int lattice, int interp, const double* fin, const double* min, const double* Pin,
                            const unsigned char* okin, double* fout, double* mout, int* niter, const Pending& pend,
                            const StepArgs& a, hipStream_t st) {
@@ -1383,16 +1492,14 @@ hipError_t launch_step_f64(int lattice, int interp, const double* fin, const dou
 #define POND_LAUNCH(L, I, B)                                                                                     \
   do {                                                                                                           \
     hipLaunchKernelGGL((k_step_f64<L, I, B>), dim3(blocks), dim3(64), 0, st, src, fout, mout, niter, pend, a);   \
-    hipLaunchKernelGGL((k_step_f64_rest<L, I, B>), dim3(blocks2), dim3(64), 0, st, src, fout, mout, niter, pend, \
-                       a);                                                                                       \
+    launch_rest<L, I, B>(blocks2, st, src, fout, mout, niter, pend, a);                                          \
   } while (0)
 #define POND_LAUNCH_R(L, I)                                                                                      \
   do {                                                                                                           \
     const unsigned w = (unsigned)(((int64_t)blocks + a.refill_chunk - 1) / a.refill_chunk);                       \
     hipLaunchKernelGGL((k_step_f64_refill<L, I, true>), dim3(w), dim3(64), 0, st, src, fout, mout, niter, pend, a, \
                        a.refill_chunk);                                                                          \
-    hipLaunchKernelGGL((k_step_f64_rest<L, I, true>), dim3(blocks2), dim3(64), 0, st, src, fout, mout, niter,    \
-                       pend, a);                                                                                 \
+    launch_rest<L, I, true>(blocks2, st, src, fout, mout, niter, pend, a);                                       \
   } while (0)
   // kernel codes: 0 WENO5 as written, 1 Lagrange-4, 2 WENO5 centred, 3 Lagrange-4 with mirror
   // boundaries (its own instantiation, so the periodic kernels carry none of the mirror code),
