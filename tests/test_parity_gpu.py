@@ -173,6 +173,33 @@ def test_edge_nan_neighbour_and_hot_spot(gpu):
     check(ref, got)
 
 
+@pytest.mark.parametrize("interp", ["weno5c", "weno5", "lagr4"])
+def test_hot_cells_take_the_one_target_path(gpu, interp):
+    """Cells whose thermal speed sqrt(T/TLat) exceeds 1.5 (here T = 1.2 and 0.9 in a T = 0.02
+    state) have their three ex targets' windows too far apart for the grouped D3Q27 step and
+    take the one-target path, inside a grid whose other cells take the grouped one: bit for bit
+    against the oracle, on a row-patch grid and on a ragged one."""
+    for shape, seed in (((64, 4, 4), 43), ((13, 6, 5), 44)):
+        rho, ux, uy, uz, T = x_dominant_state(shape, seed=seed, T0=0.02) if interp == "weno5" else \
+            smooth_random_state(shape, seed=seed, T0=0.02)
+        T = T.copy()
+        T[[9, 40, 41]] = [1.2, 0.9, 1.2]
+        print(f"[parity] hot cells {interp} {shape}", flush=True)
+        o = po.Oracle("D3Q27", shape, interp)
+        o.set_state(rho, ux, uy, uz, T)
+        o.step(3)
+        ref = o.fields()
+        with Pond("D3Q27", shape, interp, "fp64") as p:
+            p.set_state(rho, ux, uy, uz, T)
+            p.step(3)
+            got = p.fields()
+        for k in FIELDS:
+            assert np.array_equal(np.isnan(got[k]), np.isnan(ref[k])), k
+            fin = np.isfinite(ref[k])
+            assert np.array_equal(got[k][fin], ref[k][fin]), k
+        assert np.array_equal(got["niter"], ref["niter"])
+
+
 def test_run_twice_bitwise(gpu):
     shape = (16, 8, 4)
     f = smooth_random_state(shape, seed=51, T0=0.01)
