@@ -614,7 +614,7 @@ __device__ void interp_pass_weno5c(const NodeSrcF& src, int x, int y, int z, dou
       int ya[3];
 #pragma unroll
       for (int e = 0; e < 3; e++) ya[e] = yr - ry[e];
-#pragma unroll 1
+#pragma unroll
       for (int b = 0; b < 3; b++) {
         const uint32_t zb = b == 0 ? zo[0] : b == 1 ? zo[1] : zo[2];
         const float fzb = b == 0 ? Fz[0] : b == 1 ? Fz[1] : Fz[2];
@@ -844,7 +844,7 @@ __device__ void interp_pass_lagr4(const NodeSrcF& src, int x, int y, int z, doub
         const int a = yr - ly[e];
         wy[e] = a == 0 ? Fy[e][0] : a == 1 ? Fy[e][1] : a == 2 ? Fy[e][2] : a == 3 ? Fy[e][3] : 0.f;
       }
-#pragma unroll 1
+#pragma unroll
       for (int b = 0; b < 4; b++) {
         bool rzf = false;
         const int zw = MIR ? mirror_wrap(lz + b, g.nz, (g.mirror & 4) != 0, rzf) : wrapi(lz + b, g.nz);
