@@ -582,34 +582,57 @@ __device__ void interp_pass_weno5c(const NodeSrcF& src, int x, int y, int z, dou
   uint32_t zo[3];
 #pragma unroll
   for (int b = 0; b < 3; b++) zo[b] = (uint32_t)wrapi(rz + b, g.nz) * ((uint32_t)g.nx * (uint32_t)g.ny);
-  // per group e and target t: the smoothness samples at the five window positions (the middle
-  // node of each slice) and the three sub-stencil sums, accumulated slice by slice
-  float ub[3][3][5], Sk[3][3][3];
+  // per group e and target t: the three sub-stencil sums, accumulated slice by slice, and the
+  // smoothness samples at the five window positions (the middle node of each slice).  The
+  // samples live in LDS, one column per lane (the kernels calling this run 64-lane blocks), at
+  // the window position the slice has for the target: every position of every target and group
+  // is written once (the slices span each target's window, the rows each group's middle row),
+  // so nothing needs clearing.
+  __shared__ float s_ub[45][64];
+  const int lane = (int)threadIdx.x;
+  float Sk[3][3][3];
 #pragma unroll
   for (int e = 0; e < 3; e++)
 #pragma unroll
-    for (int t = 0; t < 3; t++) {
-#pragma unroll
-      for (int j = 0; j < 5; j++) ub[e][t][j] = 0.f;
+    for (int t = 0; t < 3; t++)
 #pragma unroll
       for (int k = 0; k < 3; k++) Sk[e][t][k] = 0.f;
-    }
   // target t's window position of slice xs is j = xs - (r_t - 2); sub-stencil k holds positions
-  // k .. k+2, with the quadratic Lagrange weights at sk = c_t + 2 - k
+  // k .. k+2, with the quadratic Lagrange weights at sk = c_t + 2 - k.
+  //
+  // The (slice, row) steps run as one loop whose node loads are one step ahead: node b of the
+  // next step (the next row of the slice, or the first row of the next slice) is loaded into
+  // the registers of node b of the current step once that node's contributions are added, so a
+  // load has the rest of the step (about two nodes of arithmetic) to arrive in.  The loads are
+  // unconditional (the step after the last loads three wrapped, unused nodes), so no lane has
+  // to keep a node it has used.  The arithmetic and its order are those of a plain slice/row
+  // loop.
+  float fb[3][27];
+  Gauge4 nb[3];
+  {
+    const uint32_t o0 = (uint32_t)wrapi(xlo, g.nx) + (uint32_t)wrapi(ylo, g.ny) * (uint32_t)g.nx;
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      const uint32_t zb = b == 0 ? zo[0] : b == 1 ? zo[1] : zo[2];
+      load27(src.f, (o0 + zb) * 4u, fb[b]);
+      nb[b] = ldg4(src, (o0 + zb) * 4u);
+    }
+  }
+  float yz[3][3];
+#pragma unroll
+  for (int e = 0; e < 3; e++)
+#pragma unroll
+    for (int t = 0; t < 3; t++) yz[e][t] = 0.f;
+  int xs = xlo, yr = ylo;
 #pragma unroll 1
-  for (int xs = xlo; xs <= xhi; xs++) {
-    const uint32_t xw = (uint32_t)wrapi(xs, g.nx);
+  while (true) {
     int jt[3];
 #pragma unroll
     for (int t = 0; t < 3; t++) jt[t] = xs - (r[t] - 2);
-    float yz[3][3];
-#pragma unroll
-    for (int e = 0; e < 3; e++)
-#pragma unroll
-      for (int t = 0; t < 3; t++) yz[e][t] = 0.f;
-#pragma unroll 1
-    for (int yr = ylo; yr <= yhi; yr++) {
-      const uint32_t yo = (uint32_t)wrapi(yr, g.ny) * (uint32_t)g.nx;
+    const bool last_row = yr == yhi;
+    const int xn = last_row ? xs + 1 : xs, yn = last_row ? ylo : yr + 1;
+    const uint32_t on = (uint32_t)wrapi(xn, g.nx) + (uint32_t)wrapi(yn, g.ny) * (uint32_t)g.nx;
+    {
       // the row's place in each group's three rows (outside: the group skips the node)
       int ya[3];
 #pragma unroll
@@ -618,10 +641,7 @@ __device__ void interp_pass_weno5c(const NodeSrcF& src, int x, int y, int z, dou
       for (int b = 0; b < 3; b++) {
         const uint32_t zb = b == 0 ? zo[0] : b == 1 ? zo[1] : zo[2];
         const float fzb = b == 0 ? Fz[0] : b == 1 ? Fz[1] : Fz[2];
-        const uint32_t boff = (xw + yo + zb) * 4u;
-        float f[27];
-        load27(src.f, boff, f);
-        const Gauge4 n = ldg4(src, boff);
+        const Gauge4 n = nb[b];
         // shared by the three groups: the z basis and contraction, the y and x differences
         float Lz[3];
         basis3u(((u1z - n.uz) + u1zl) * sqTb, n.sq, czb, Tzb, Lz);
@@ -632,7 +652,7 @@ __device__ void interp_pass_weno5c(const NodeSrcF& src, int x, int y, int z, dou
           for (int jb = 0; jb < 3; jb++) {
             float sz = 0.f;
 #pragma unroll
-            for (int jc = 0; jc < 3; jc++) sz = __builtin_fmaf(Lz[jc], f[kIdx27.v[ja][jb][jc]], sz);
+            for (int jc = 0; jc < 3; jc++) sz = __builtin_fmaf(Lz[jc], fb[b][kIdx27.v[ja][jb][jc]], sz);
             szz[ja][jb] = sz;
           }
         // the x basis of the three targets is A_j (A_j - c) or A_j^2 - T1 with A_j = Dx - (j-1) sq0:
@@ -673,25 +693,36 @@ __device__ void interp_pass_weno5c(const NodeSrcF& src, int x, int y, int z, dou
           if (b == 1 && a == 1) {
 #pragma unroll
             for (int t = 0; t < 3; t++)
-#pragma unroll
-              for (int jj = 0; jj < 5; jj++) ub[e][t][jj] = jt[t] == jj ? Tt[t] : ub[e][t][jj];
+              if (jt[t] >= 0 && jt[t] <= 4) s_ub[(e * 3 + t) * 5 + jt[t]][lane] = Tt[t];
           }
         }
+        // node b of this step is used up: load node b of the next step in its place
+        load27(src.f, (on + zb) * 4u, fb[b]);
+        nb[b] = ldg4(src, (on + zb) * 4u);
       }
     }
-    // the slice's y/z sums into the sub-stencil sums whose window holds it
+    if (last_row) {
+      // the slice's y/z sums into the sub-stencil sums whose window holds it
 #pragma unroll
-    for (int t = 0; t < 3; t++) {
+      for (int t = 0; t < 3; t++) {
 #pragma unroll
-      for (int k = 0; k < 3; k++) {
-        const float sk = c[t] + (float)(2 - k);
-        const int m = jt[t] - k;  // node of sub-stencil k, 0..2 inside it
-        const float wk = m == 0 ? 0.5f * (sk - 1.0f) * (sk - 2.0f) : m == 1 ? -sk * (sk - 2.0f) : 0.5f * sk * (sk - 1.0f);
-        const bool in = m >= 0 && m <= 2;
+        for (int k = 0; k < 3; k++) {
+          const float sk = c[t] + (float)(2 - k);
+          const int m = jt[t] - k;  // node of sub-stencil k, 0..2 inside it
+          const float wk = m == 0 ? 0.5f * (sk - 1.0f) * (sk - 2.0f) : m == 1 ? -sk * (sk - 2.0f) : 0.5f * sk * (sk - 1.0f);
+          const bool in = m >= 0 && m <= 2;
 #pragma unroll
-        for (int e = 0; e < 3; e++) Sk[e][t][k] = in ? __builtin_fmaf(wk, yz[e][t], Sk[e][t][k]) : Sk[e][t][k];
+          for (int e = 0; e < 3; e++) Sk[e][t][k] = in ? __builtin_fmaf(wk, yz[e][t], Sk[e][t][k]) : Sk[e][t][k];
+        }
       }
+#pragma unroll
+      for (int e = 0; e < 3; e++)
+#pragma unroll
+        for (int t = 0; t < 3; t++) yz[e][t] = 0.f;
+      if (xn > xhi) break;
     }
+    xs = xn;
+    yr = yn;
   }
   // as in interp_group_weno5c: the basis factors, the sub-stencil sums, the WENO combination
 #pragma unroll
@@ -702,7 +733,7 @@ __device__ void interp_pass_weno5c(const NodeSrcF& src, int x, int y, int z, dou
       const float wt = (t != 1 ? h : mm) * wyz;
       float ubt[5], S[3];
 #pragma unroll
-      for (int j = 0; j < 5; j++) ubt[j] = ub[e][t][j] * wt;
+      for (int j = 0; j < 5; j++) ubt[j] = s_ub[(e * 3 + t) * 5 + j][lane] * wt;
 #pragma unroll
       for (int k = 0; k < 3; k++) S[k] = Sk[e][t][k] * wt;
       fout[split27_fidx(kIdx27.v[t][e][ez + 1], cell, P)] = weno5_combine(c[t], ubt, S);
@@ -823,97 +854,123 @@ __device__ void interp_pass_lagr4(const NodeSrcF& src, int x, int y, int z, doub
   for (int e = 0; e < 3; e++)
 #pragma unroll
     for (int t = 0; t < 3; t++) acc[e][t] = 0.f;
+  // byte offset of node (xs, yr, lz + b), its mirror image where it lies beyond a mirrored face
+  auto node_off = [&](int xs_, int yr_, int b) -> uint32_t {
+    bool rv;
+    const int xw = MIR ? mirror_wrap(xs_, g.nx, (g.mirror & 1) != 0, rv) : wrapi(xs_, g.nx);
+    const int yw = MIR ? mirror_wrap(yr_, g.ny, (g.mirror & 2) != 0, rv) : wrapi(yr_, g.ny);
+    const int zw = MIR ? mirror_wrap(lz + b, g.nz, (g.mirror & 4) != 0, rv) : wrapi(lz + b, g.nz);
+    return (uint32_t)cell_index(g, xw, yw, zw) * 4u;
+  };
+  // The (slice, row) steps run as one loop whose node loads are one step ahead, as in
+  // interp_pass_weno5c: node b of the next step is loaded into the registers of node b of the
+  // current step once its contributions are added (unconditionally: the step after the last
+  // loads four wrapped, unused nodes).  The arithmetic and its order are those of a plain
+  // slice/row loop.
+  float fb[4][27];
+  Gauge4 nb[4];
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    const uint32_t boff = node_off(xlo, ylo, b);
+    load27(src.f, boff, fb[b]);
+    nb[b] = ldg4(src, boff);
+  }
+  int xs = xlo, yr = ylo;
 #pragma unroll 1
-  for (int xs = xlo; xs <= xhi; xs++) {
-    bool rx = false;
-    const int xw = MIR ? mirror_wrap(xs, g.nx, (g.mirror & 1) != 0, rx) : wrapi(xs, g.nx);
-    // the slice's Lagrange weight in each target's four x slices (0 outside)
-    float wx[3];
+  while (true) {
+    bool rx = false, ryf = false;
+    if (MIR) {
+      mirror_wrap(xs, g.nx, (g.mirror & 1) != 0, rx);
+      mirror_wrap(yr, g.ny, (g.mirror & 2) != 0, ryf);
+    }
+    // the slice's Lagrange weight in each target's four x slices (0 outside), the row's in each
+    // group's four rows
+    float wx[3], wy[3];
 #pragma unroll
     for (int t = 0; t < 3; t++) {
       const int m = xs - lx[t];
       wx[t] = m == 0 ? Fx[t][0] : m == 1 ? Fx[t][1] : m == 2 ? Fx[t][2] : m == 3 ? Fx[t][3] : 0.f;
     }
-#pragma unroll 1
-    for (int yr = ylo; yr <= yhi; yr++) {
-      bool ryf = false;
-      const int yw = MIR ? mirror_wrap(yr, g.ny, (g.mirror & 2) != 0, ryf) : wrapi(yr, g.ny);
-      float wy[3];
 #pragma unroll
-      for (int e = 0; e < 3; e++) {
-        const int a = yr - ly[e];
-        wy[e] = a == 0 ? Fy[e][0] : a == 1 ? Fy[e][1] : a == 2 ? Fy[e][2] : a == 3 ? Fy[e][3] : 0.f;
+    for (int e = 0; e < 3; e++) {
+      const int a = yr - ly[e];
+      wy[e] = a == 0 ? Fy[e][0] : a == 1 ? Fy[e][1] : a == 2 ? Fy[e][2] : a == 3 ? Fy[e][3] : 0.f;
+    }
+    const bool last_row = yr == yhi;
+    const int xn = last_row ? xs + 1 : xs, yn = last_row ? ylo : yr + 1;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      bool rzf = false;
+      if (MIR) mirror_wrap(lz + b, g.nz, (g.mirror & 4) != 0, rzf);
+      const float fzb = b == 0 ? Fz[0] : b == 1 ? Fz[1] : b == 2 ? Fz[2] : Fz[3];
+      Gauge4 n = nb[b];
+      // a node beyond a mirrored face is the mirror image of the cell inside
+      // (Cell::mirrX/Y/Z, structs.cuh:552-578): its axis velocity negated, and its populations
+      // exchanged with their mirror images, which is the same as reversing that axis's basis
+      // values (node_xfer_mirrored)
+      if (MIR) {
+        n.ux = rx ? -n.ux : n.ux;
+        n.uy = ryf ? -n.uy : n.uy;
+        n.uz = rzf ? -n.uz : n.uz;
       }
+      auto flip = [](bool m, float L[3]) {
+        const float a = L[0];
+        L[0] = m ? L[2] : L[0];
+        L[2] = m ? a : L[2];
+      };
+      float Lz[3];
+      basis3u(((u1z - n.uz) + u1zl) * sqTb, n.sq, czb, Tzb, Lz);
+      if (MIR) flip(rzf, Lz);
+      float szz[3][3];
 #pragma unroll
-      for (int b = 0; b < 4; b++) {
-        bool rzf = false;
-        const int zw = MIR ? mirror_wrap(lz + b, g.nz, (g.mirror & 4) != 0, rzf) : wrapi(lz + b, g.nz);
-        const float fzb = b == 0 ? Fz[0] : b == 1 ? Fz[1] : b == 2 ? Fz[2] : Fz[3];
-        const uint32_t boff = (uint32_t)cell_index(g, xw, yw, zw) * 4u;
-        float f[27];
-        load27(src.f, boff, f);
-        Gauge4 n = ldg4(src, boff);
-        // a node beyond a mirrored face is the mirror image of the cell inside
-        // (Cell::mirrX/Y/Z, structs.cuh:552-578): its axis velocity negated, and its populations
-        // exchanged with their mirror images, which is the same as reversing that axis's basis
-        // values (node_xfer_mirrored)
-        if (MIR) {
-          n.ux = rx ? -n.ux : n.ux;
-          n.uy = ryf ? -n.uy : n.uy;
-          n.uz = rzf ? -n.uz : n.uz;
-        }
-        auto flip = [](bool m, float L[3]) {
-          const float a = L[0];
-          L[0] = m ? L[2] : L[0];
-          L[2] = m ? a : L[2];
-        };
-        float Lz[3];
-        basis3u(((u1z - n.uz) + u1zl) * sqTb, n.sq, czb, Tzb, Lz);
-        if (MIR) flip(rzf, Lz);
-        float szz[3][3];
-#pragma unroll
-        for (int ja = 0; ja < 3; ja++)
-#pragma unroll
-          for (int jb = 0; jb < 3; jb++) {
-            float sz = 0.f;
-#pragma unroll
-            for (int jc = 0; jc < 3; jc++) sz = __builtin_fmaf(Lz[jc], f[kIdx27.v[ja][jb][jc]], sz);
-            szz[ja][jb] = sz;
-          }
-        const float Dy = ((u1y - n.uy) + u1yl) * sqTb;
-        const float Dx = ((u1x - n.ux) + u1xl) * sqTb;
-        float A0 = Dx + n.sq, A1 = Dx, A2 = Dx - n.sq;
-        if (MIR) {
-          const float a = A0;
-          A0 = rx ? A2 : A0;
-          A2 = rx ? a : A2;
-        }
-        // x moments of the z sums, common to the three groups (as in interp_pass_weno5c)
-        const float B0 = A0 * A0, B1 = A1 * A1, B2 = A2 * A2;
-        float xr[3], xq[3], xp[3];
+      for (int ja = 0; ja < 3; ja++)
 #pragma unroll
         for (int jb = 0; jb < 3; jb++) {
-          xr[jb] = (szz[0][jb] + szz[1][jb]) + szz[2][jb];
-          xq[jb] = __builtin_fmaf(A2, szz[2][jb], __builtin_fmaf(A1, szz[1][jb], A0 * szz[0][jb]));
-          xp[jb] = __builtin_fmaf(B2, szz[2][jb], __builtin_fmaf(B1, szz[1][jb], B0 * szz[0][jb]));
-        }
+          float sz = 0.f;
 #pragma unroll
-        for (int e = 0; e < 3; e++) {
-          if (wy[e] == 0.f) continue;
-          const int ey = e - 1;
-          float Ly[3];
-          basis3u(Dy, n.sq, ey ? (float)ey * sq1 : 0.0f, ey ? 0.0f : T1, Ly);
-          if (MIR) flip(ryf, Ly);
-          const float R = __builtin_fmaf(Ly[2], xr[2], __builtin_fmaf(Ly[1], xr[1], Ly[0] * xr[0]));
-          const float Q = __builtin_fmaf(Ly[2], xq[2], __builtin_fmaf(Ly[1], xq[1], Ly[0] * xq[0]));
-          const float Pp = __builtin_fmaf(Ly[2], xp[2], __builtin_fmaf(Ly[1], xp[1], Ly[0] * xp[0]));
-          const float Tt[3] = {__builtin_fmaf(sq1, Q, Pp), __builtin_fmaf(-T1, R, Pp), __builtin_fmaf(-sq1, Q, Pp)};
-          const float wyz = wy[e] * fzb;
-#pragma unroll
-          for (int t = 0; t < 3; t++) acc[e][t] = __builtin_fmaf(wx[t] * wyz, Tt[t], acc[e][t]);
+          for (int jc = 0; jc < 3; jc++) sz = __builtin_fmaf(Lz[jc], fb[b][kIdx27.v[ja][jb][jc]], sz);
+          szz[ja][jb] = sz;
         }
+      const float Dy = ((u1y - n.uy) + u1yl) * sqTb;
+      const float Dx = ((u1x - n.ux) + u1xl) * sqTb;
+      float A0 = Dx + n.sq, A1 = Dx, A2 = Dx - n.sq;
+      if (MIR) {
+        const float a = A0;
+        A0 = rx ? A2 : A0;
+        A2 = rx ? a : A2;
       }
+      // x moments of the z sums, common to the three groups (as in interp_pass_weno5c)
+      const float B0 = A0 * A0, B1 = A1 * A1, B2 = A2 * A2;
+      float xr[3], xq[3], xp[3];
+#pragma unroll
+      for (int jb = 0; jb < 3; jb++) {
+        xr[jb] = (szz[0][jb] + szz[1][jb]) + szz[2][jb];
+        xq[jb] = __builtin_fmaf(A2, szz[2][jb], __builtin_fmaf(A1, szz[1][jb], A0 * szz[0][jb]));
+        xp[jb] = __builtin_fmaf(B2, szz[2][jb], __builtin_fmaf(B1, szz[1][jb], B0 * szz[0][jb]));
+      }
+#pragma unroll
+      for (int e = 0; e < 3; e++) {
+        if (wy[e] == 0.f) continue;
+        const int ey = e - 1;
+        float Ly[3];
+        basis3u(Dy, n.sq, ey ? (float)ey * sq1 : 0.0f, ey ? 0.0f : T1, Ly);
+        if (MIR) flip(ryf, Ly);
+        const float R = __builtin_fmaf(Ly[2], xr[2], __builtin_fmaf(Ly[1], xr[1], Ly[0] * xr[0]));
+        const float Q = __builtin_fmaf(Ly[2], xq[2], __builtin_fmaf(Ly[1], xq[1], Ly[0] * xq[0]));
+        const float Pp = __builtin_fmaf(Ly[2], xp[2], __builtin_fmaf(Ly[1], xp[1], Ly[0] * xp[0]));
+        const float Tt[3] = {__builtin_fmaf(sq1, Q, Pp), __builtin_fmaf(-T1, R, Pp), __builtin_fmaf(-sq1, Q, Pp)};
+        const float wyz = wy[e] * fzb;
+#pragma unroll
+        for (int t = 0; t < 3; t++) acc[e][t] = __builtin_fmaf(wx[t] * wyz, Tt[t], acc[e][t]);
+      }
+      // node b of this step is used up: load node b of the next step in its place
+      const uint32_t boff = node_off(xn, yn, b);
+      load27(src.f, boff, fb[b]);
+      nb[b] = ldg4(src, boff);
     }
+    if (last_row && xn > xhi) break;
+    xs = xn;
+    yr = yn;
   }
 #pragma unroll
   for (int e = 0; e < 3; e++) {
@@ -1172,8 +1229,13 @@ __device__ __forceinline__ void split27_pass1(const float* __restrict__ fin, con
   }
 }
 
+// The D3Q27 split kernels are held to at least two waves per SIMD (256 registers): the WENO5C
+// and Lagrange-4 passes, with their node loads one step ahead, would otherwise take a few more
+// registers and drop to one wave, which leaves the load latency unhidden; held to two, they keep
+// a few rarely used values in scratch instead.
+#define POND_SPLIT27_KERNEL __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
 template <int INTERP>
-__global__ __launch_bounds__(64) void k_step_split27(const float* __restrict__ fin, const float* __restrict__ min,
+POND_SPLIT27_KERNEL void k_step_split27(const float* __restrict__ fin, const float* __restrict__ min,
                                                      const double* __restrict__ Tin, const float* __restrict__ g4in,
                                                      float* __restrict__ fout, float* __restrict__ mout,
                                                      double* __restrict__ Tout, int* __restrict__ niter_out,
@@ -1181,6 +1243,7 @@ __global__ __launch_bounds__(64) void k_step_split27(const float* __restrict__ f
                                                      const SplitArgs a) {
   split27_pass1<INTERP>(fin, min, Tin, g4in, fout, mout, Tout, niter_out, guess, pend, a);
 }
+
 
 // Single pass with lane refill.  A wave owns a chunk of `chunk` consecutive 64-cell tiles.  Its
 // lanes start on the first tile; whenever lanes finish their cells, they take the chunk's next
@@ -1237,21 +1300,22 @@ __device__ __forceinline__ void split27_refill(const float* __restrict__ fin, co
 }
 
 template <int INTERP>
-__global__ __launch_bounds__(64) void k_step_split27_refill(
+POND_SPLIT27_KERNEL void k_step_split27_refill(
     const float* __restrict__ fin, const float* __restrict__ min, const double* __restrict__ Tin,
     const float* __restrict__ g4in, float* __restrict__ fout, float* __restrict__ mout, double* __restrict__ Tout,
     int* __restrict__ niter_out, const float* __restrict__ guess, const SplitArgs a, int chunk) {
   split27_refill<INTERP>(fin, min, Tin, g4in, fout, mout, Tout, niter_out, guess, a, chunk);
 }
 
+
 // Second pass: a fixed grid of waves walks the pending list; every cell resumes its iteration
 // where the first pass left it.
 template <int INTERP>
-__global__ __launch_bounds__(64) void k_step_split27_rest(const float* __restrict__ fin, const float* __restrict__ min,
-                                                          const double* __restrict__ Tin, const float* __restrict__ g4in,
-                                                          float* __restrict__ fout, float* __restrict__ mout,
-                                                          double* __restrict__ Tout, int* __restrict__ niter_out,
-                                                          SplitPending pend, const SplitArgs a) {
+__device__ __forceinline__ void split27_rest(const float* __restrict__ fin, const float* __restrict__ min,
+                                             const double* __restrict__ Tin, const float* __restrict__ g4in,
+                                             float* __restrict__ fout, float* __restrict__ mout,
+                                             double* __restrict__ Tout, int* __restrict__ niter_out,
+                                             const SplitPending& pend, const SplitArgs& a) {
   const Grid& g = a.g;
   const int count = *pend.count;
   const NodeSrcF src = make_src(fin, g4in, g);
@@ -1276,6 +1340,16 @@ __global__ __launch_bounds__(64) void k_step_split27_rest(const float* __restric
     split27_finish(fout, mout, Tout, niter_out, cell, st, a);
   }
 }
+
+template <int INTERP>
+POND_SPLIT27_KERNEL void k_step_split27_rest(const float* __restrict__ fin, const float* __restrict__ min,
+                                                          const double* __restrict__ Tin, const float* __restrict__ g4in,
+                                                          float* __restrict__ fout, float* __restrict__ mout,
+                                                          double* __restrict__ Tout, int* __restrict__ niter_out,
+                                                          SplitPending pend, const SplitArgs a) {
+  split27_rest<INTERP>(fin, min, Tin, g4in, fout, mout, Tout, niter_out, pend, a);
+}
+
 
 __global__ void k_zero_count(int* c) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0;
