@@ -84,7 +84,8 @@ typedef struct {
      tile_patch    cells are taken in 64-cell x-row segments whose (y, z) rows are visited in
                    tile_patch x tile_patch patches, so the waves resident on one XCD share
                    their stencil halo in its L2; used where nx % 64 == 0 and tile_patch divides
-                   ny and nz.  0: plain row order.  Default 4.
+                   ny and nz.  0: plain row order.  -1 (default): 8 for the POND_SPLIT
+                   D3Q27 WENO5C step, 4 otherwise.
      refill_chunk  64-cell tiles owned by one wave, whose lanes take the next unstarted cell
                    of the chunk when theirs converges.  0: one tile per wave.  -1 (default):
                    FP64 2 tiles; split the whole x row with at least 4 tiles.

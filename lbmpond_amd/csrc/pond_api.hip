@@ -263,10 +263,12 @@ void pond_params_default(pond_params* p, int lattice) {
   // residual; 1e-7 converges reliably above it
   p->split_tol_rel = 1e-7;
   p->split_tol_abs = 1e-12;
-  // Row patches, 384^3 shock-bubble D3Q27 WENO5C: split step 1124 ms in row order, 1025 ms with
-  // 4x4 patches, 1063-1075 ms with 8, 16 or 32; the FP64 step (pair planes, lane refill) 4.10 s
-  // in row order, 3.13 s with 4x4 patches, 3.14 s with 8x8.
-  p->tile_patch = 4;
+  // Row patches (-1: chosen in pond_create).  384^3 shock-bubble D3Q27: the FP64 step (pair
+  // planes, lane refill) 4.10 s in row order, 3.13 s with 4x4 patches, 3.14 s with 8x8; the split
+  // step with its node loads one step ahead, WENO5C 261 ms in row order, 161.4 ms with 4x4
+  // patches, 157.0 with 8x8, 157.8 with 6x6, 157.2 with 12x12, 158.2 with 16x16; Lagrange-4
+  // 422, 225.2, 226.2 and 232.2 ms (row order, 4, 8, 16).
+  p->tile_patch = -1;
   p->refill_chunk = -1;
   p->pass1_iters = -1;
   p->global_loads = 0;
@@ -326,7 +328,7 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
   g.nx = (int)p->nx; g.ny = (int)p->ny; g.nz = (int)p->nz;
   g.n = p->nx * p->ny * p->nz;
   g.plane = round_up(g.n, 64);
-  if (p->tile_patch < 0) return bail(POND_ERR_INVALID, "pond_create: tile_patch must be >= 0");
+  if (p->tile_patch < -1) return bail(POND_ERR_INVALID, "pond_create: tile_patch must be >= -1");
   if (p->refill_chunk < -1) return bail(POND_ERR_INVALID, "pond_create: refill_chunk must be >= -1");
   if (p->pass1_iters < -1 || p->pass1_iters == 0)
     return bail(POND_ERR_INVALID, "pond_create: pass1_iters must be -1 or >= 1");
@@ -340,8 +342,11 @@ pond_status pond_create(const pond_params* p, pond_ctx** out) {
                  p->nx > amr_max_line()))
     return bail(POND_ERR_INVALID, "pond_create: amr applies to POND_FP64 D1Q5 lines (ny = nz = 1) of up to " +
                                       std::to_string(amr_max_line()) + " cells");
-  // row patches where the grid divides into them (ignored elsewhere)
-  const int rp = p->tile_patch;
+  // row patches where the grid divides into them (ignored elsewhere); by default 8x8 for the
+  // split D3Q27 WENO5C step, 4x4 for the others
+  const int rp = p->tile_patch >= 0 ? p->tile_patch
+                 : (p->precision == POND_SPLIT && p->lattice == POND_D3Q27 && p->interp == POND_WENO5C) ? 8
+                                                                                                        : 4;
   g.rp = (rp > 0 && g.nx % 64 == 0 && g.ny % rp == 0 && g.nz % rp == 0) ? rp : 0;
   g.ntiles = g.rp > 0 ? (int64_t)(g.nx / 64) * g.ny * g.nz : (g.n + 63) / 64;
   g.mirror = p->mirror;
