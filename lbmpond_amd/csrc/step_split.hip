@@ -1003,7 +1003,8 @@ __device__ __forceinline__ void split_prev_gauge(const float* __restrict__ min, 
 // One fixed-point iteration of one cell: true when it has converged.
 template <int INTERP>
 __device__ __forceinline__ bool split27_once(const NodeSrcF& src, float* __restrict__ fout, int64_t cell, int x,
-                                             int y, int z, const double gp[4], SplitIter& st, const SplitArgs& a) {
+                                             int y, int z, const float* __restrict__ min,
+                                             const double* __restrict__ Tin, SplitIter& st, const SplitArgs& a) {
   const int64_t P = a.g.plane;
   const float sqTb = (float)a.sqrt_tbase;
   double* gc = st.gc;
@@ -1098,7 +1099,11 @@ __device__ __forceinline__ bool split27_once(const NodeSrcF& src, float* __restr
     }
     if (st.learned && (res != res || (st.niter > 1 && !(res < st.res_prev)) || st.niter == kLearnedBudget)) {
       // the predicted start does not contract (or not within the budget): restart from the
-      // previous gauge, so the cell ends where the plain method ends
+      // previous gauge (read again from the step's input, which the step does not write, so
+      // no lane carries it through its iterations), so the cell ends where the plain method
+      // ends
+      double gp[4], rho0;
+      split_prev_gauge(min, Tin, P, cell, a.tlat, gp, rho0);
       for (int k = 0; k < 4; k++) gc[k] = gp[k];
       st.learned = false;
       st.contracted = false;
@@ -1119,9 +1124,10 @@ __device__ __forceinline__ bool split27_once(const NodeSrcF& src, float* __restr
 // are left in fout.
 template <int INTERP>
 __device__ bool split27_iterate(const NodeSrcF& src, float* __restrict__ fout, int64_t cell, int x, int y, int z,
-                                const double gp[4], SplitIter& st, int stop_at, const SplitArgs& a) {
+                                const float* __restrict__ min, const double* __restrict__ Tin, SplitIter& st,
+                                int stop_at, const SplitArgs& a) {
   while (st.niter < stop_at && st.niter - st.base < a.max_iter)
-    if (split27_once<INTERP>(src, fout, cell, x, y, z, gp, st, a)) return true;
+    if (split27_once<INTERP>(src, fout, cell, x, y, z, min, Tin, st, a)) return true;
   return false;
 }
 
@@ -1164,9 +1170,9 @@ __device__ __forceinline__ void split27_finish(float* __restrict__ fout, float* 
 // which this step overwrites cell by cell after reading).
 __device__ __forceinline__ void split27_start(const float* __restrict__ min, const double* __restrict__ Tin,
                                               const float* __restrict__ guess, const int* __restrict__ niter_prev,
-                                              int64_t cell, double gp[4], SplitIter& st, const SplitArgs& a) {
+                                              int64_t cell, SplitIter& st, const SplitArgs& a) {
   const int64_t P = a.g.plane;
-  double rho0;
+  double gp[4], rho0;
   split_prev_gauge(min, Tin, P, cell, a.tlat, gp, rho0);
   for (int k = 0; k < 4; k++) st.gc[k] = gp[k];
   st.res_prev = 0;
@@ -1211,10 +1217,9 @@ __device__ __forceinline__ void split27_pass1(const float* __restrict__ fin, con
   SplitIter st;
   const int64_t cell = live ? cell_index(g, x, y, z) : 0;
   if (live) {
-    double gp[4];
-    split27_start(min, Tin, guess, niter_out, cell, gp, st, a);
+    split27_start(min, Tin, guess, niter_out, cell, st, a);
     const NodeSrcF src = make_src(fin, g4in, g);
-    const bool conv = split27_iterate<INTERP>(src, fout, cell, x, y, z, gp, st, a.pass1_iters, a);
+    const bool conv = split27_iterate<INTERP>(src, fout, cell, x, y, z, min, Tin, st, a.pass1_iters, a);
     pending = !conv && st.niter - st.base < a.max_iter;
     if (!pending) split27_finish(fout, mout, Tout, niter_out, cell, st, a);
   }
@@ -1270,7 +1275,6 @@ __device__ __forceinline__ void split27_refill(const float* __restrict__ fin, co
   bool active = false;
   int64_t cell = 0;
   int x = 0, y = 0, z = 0;
-  double gp[4] = {0, 0, 0, 0};
   SplitIter st;
   while (true) {
     // lanes without a cell take the next slots, in lane order; a slot outside the grid (ragged
@@ -1281,7 +1285,7 @@ __device__ __forceinline__ void split27_refill(const float* __restrict__ fin, co
       const int64_t slot = next + rank;
       if (slot < total && tile_cell(g, t0 + slot / 64, (int)(slot % 64), x, y, z)) {
         cell = cell_index(g, x, y, z);
-        split27_start(min, Tin, guess, niter_out, cell, gp, st, a);
+        split27_start(min, Tin, guess, niter_out, cell, st, a);
         active = true;
       }
     }
@@ -1290,7 +1294,8 @@ __device__ __forceinline__ void split27_refill(const float* __restrict__ fin, co
     if (active) {
       // the loop condition of split27_iterate, one iteration per trip
       bool done = st.niter - st.base >= a.max_iter;
-      if (!done) done = split27_once<INTERP>(src, fout, cell, x, y, z, gp, st, a) || st.niter - st.base >= a.max_iter;
+      if (!done)
+        done = split27_once<INTERP>(src, fout, cell, x, y, z, min, Tin, st, a) || st.niter - st.base >= a.max_iter;
       if (done) {
         split27_finish(fout, mout, Tout, niter_out, cell, st, a);
         active = false;
@@ -1336,7 +1341,7 @@ __device__ __forceinline__ void split27_rest(const float* __restrict__ fin, cons
     st.base = (meta >> 8) & 0xff;
     st.learned = (meta & 0x10000) != 0;
     st.contracted = (meta & 0x20000) != 0;
-    split27_iterate<INTERP>(src, fout, cell, x, y, z, gp, st, 0x7fffffff, a);
+    split27_iterate<INTERP>(src, fout, cell, x, y, z, min, Tin, st, 0x7fffffff, a);
     split27_finish(fout, mout, Tout, niter_out, cell, st, a);
   }
 }
