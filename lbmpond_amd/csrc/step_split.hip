@@ -1181,7 +1181,8 @@ __device__ __forceinline__ void split27_start(const float* __restrict__ min, con
   st.base = 0;
   st.learned = false;
   st.contracted = false;
-  if (guess) {
+  // (the previous count is tested first, so the prediction is read only where it may be taken)
+  if (guess && niter_prev[cell] >= kLearnedPrevMin && niter_prev[cell] <= kLearnedPrevMax) {
     // decode the predicted increment: (du/sqrt(T), dT/T) relative to the previous gauge
     const double sq = sqrt(gp[3]);
     const float p0 = guess[cell], p1 = guess[P + cell], p2 = guess[2 * P + cell], p3 = guess[3 * P + cell];
@@ -1189,7 +1190,7 @@ __device__ __forceinline__ void split27_start(const float* __restrict__ min, con
     const float pmax = fmaxf(fmaxf(fabsf(p0), fabsf(p1)), fmaxf(fabsf(p2), fabsf(p3)));
     // (a prediction of no change at all is the previous-gauge start, with its stopping rules)
     if (isfinite(q[0]) && isfinite(q[1]) && isfinite(q[2]) && isfinite(q[3]) && q[3] > 0 && pmax <= a.learned_trust &&
-        pmax > 0.0f && niter_prev[cell] >= kLearnedPrevMin && niter_prev[cell] <= kLearnedPrevMax) {
+        pmax > 0.0f) {
       for (int k = 0; k < 4; k++) st.gc[k] = q[k];
       st.learned = true;
     }
@@ -1494,34 +1495,48 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
       bw = (float)(1.0 / den);
     }
     float acc = 0.f, accy = 0.f, accz = 0.f;
-#pragma unroll 1
-    for (int n = 0; n < 27; n++) {
-      const int pb = n & 1;
+    // basis values of node n (lanes 0..74) into sL[n & 1]
+    auto basis = [&](int n) {
       if (k < 75) {
         const float du = (su[n][bd] - u1[bd]) - u1l[bd];
         const float bv = du + (float)kA5d[bj] * ss0[n];
-        sL[pb][bd][bk][bj] = bw * ((bv - bn[0]) * (bv - bn[1]) * (bv - bn[2]) * (bv - bn[3]));
+        sL[n & 1][bd][bk][bj] = bw * ((bv - bn[0]) * (bv - bn[1]) * (bv - bn[2]) * (bv - bn[3]));
       }
-      __syncthreads();
-      if (lane_ok) {  // tz(jx, jy, kz) at index 25 jx + 5 jy + kz
+    };
+    // z sums of node n: tz(jx, jy, kz) at index 25 jx + 5 jy + kz
+    auto zsums = [&](int n) {
+      if (lane_ok) {
         float tz = 0.f;
 #pragma unroll
-        for (int jz = 0; jz < 5; jz++) tz = __builtin_fmaf(sL[pb][2][bj][jz], sf[n][25 * bd + 5 * bk + jz], tz);
+        for (int jz = 0; jz < 5; jz++) tz = __builtin_fmaf(sL[n & 1][2][bj][jz], sf[n][25 * bd + 5 * bk + jz], tz);
         sTz[k] = tz;
       }
-      __syncthreads();
+    };
+    // The three phases of consecutive nodes overlap, two barriers per node: the y sums of node
+    // n with the basis values of node n + 1 (the other sL slot), then the x sums of node n with
+    // the z sums of node n + 1 (sTz, whose node-n values the y sums have read).  Each value is
+    // the same sequence of operations as with the phases of one node at a time.
+    basis(0);
+    __syncthreads();
+    zsums(0);
+    __syncthreads();
+#pragma unroll 1
+    for (int n = 0; n < 27; n++) {
+      const int pb = n & 1;
       if (lane_ok) {  // ty(jx, ky, kz) at index 25 jx + 5 ky + kz
         float ty = 0.f;
 #pragma unroll
         for (int jy = 0; jy < 5; jy++) ty = __builtin_fmaf(sL[pb][1][bk][jy], sTz[25 * bd + 5 * jy + bj], ty);
         sTy[k] = ty;
       }
+      if (n + 1 < 27) basis(n + 1);
       __syncthreads();
       float tx = 0.f;
       if (lane_ok) {
 #pragma unroll
         for (int jx = 0; jx < 5; jx++) tx = __builtin_fmaf(sL[pb][0][X[0]][jx], sTy[25 * jx + 5 * X[1] + X[2]], tx);
       }
+      if (n + 1 < 27) zsums(n + 1);
       // (the stencil sums of the fixed 3x3x3 neighbourhood: z inner, then y, then x)
       const int ia = n / 9, ib = (n / 3) % 3, ic = n % 3;
       accz = __builtin_fmaf(Fz[ic], tx, accz);
@@ -1533,6 +1548,7 @@ __global__ __launch_bounds__(128) void k_step_split125(const float* __restrict__
           accy = 0.f;
         }
       }
+      __syncthreads();
     }
     fn = lane_ok ? acc : 0.f;
     // moments in the frame of the current gauge, FP64 block sums
