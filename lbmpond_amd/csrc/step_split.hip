@@ -1251,36 +1251,75 @@ POND_SPLIT27_KERNEL void k_step_split27(const float* __restrict__ fin, const flo
 }
 
 
-// Single pass with lane refill.  A wave owns a chunk of `chunk` consecutive 64-cell tiles.  Its
-// lanes start on the first tile; whenever lanes finish their cells, they take the chunk's next
-// unstarted cells in order (one ballot per iteration), so a lane is idle only at the end of the
-// chunk instead of for as long as the slowest cell of its tile iterates.  On the shock-bubble
-// the mean gauge iterations per cell are about 1.3 while the slowest cell of a 64-cell tile
-// needs 3.5-5, so lanes of the one-tile-per-wave step sit idle most of the time.  Each cell's
-// arithmetic is the same as in k_step_split27 (bitwise the same results); the active cells of
-// a wave stay within a window of the chunk, so node loads remain close together.
+// Persistent refill waves (below) for WENO5C only: 384^3 shock-bubble step 157.8 -> 155.2 ms;
+// the Lagrange-4 step took 294 ms against 224 with one chunk per workgroup.
+constexpr bool kRefillPersistent[4] = {false, false, true, false};
+
+// Single pass with lane refill.  Cells are dealt in chunks of `chunk` consecutive 64-cell tiles
+// (a whole x row by default).  Whenever lanes of a wave finish their cells, they take the next
+// unstarted cells of its chunk in order (one ballot per iteration), so a lane is idle only when
+// no unstarted cell is left.  On the shock-bubble the mean gauge iterations per cell are about
+// 1.3 while the slowest cell of a 64-cell tile needs 3.5-5, so lanes of the one-tile-per-wave
+// step sit idle most of the time.
+//
+// For WENO5C the waves are persistent: the grid is what the GPU holds at once, and a wave whose chunk has
+// no unstarted cell left claims the next chunk while its lanes still finish the old one, so the
+// tail of a chunk (the lanes of its last, slowest cells) is filled from the next one.  With one
+// chunk per wave, 30-40 % of the lane-iterations of the 384^3 shock-bubble step were such tails.
+// The chunks are split into eight contiguous slabs, one per XCD (workgroups are dealt to the XCDs
+// in turn, so a wave's XCD is its block index mod 8), and the waves of XCD k claim the chunks of
+// slab k in order from the counter claim[k] (zeroed before the launch), so the waves resident on
+// one XCD work on neighbouring rows and share their stencil halo in its L2.  Each cell's
+// arithmetic is the same as in k_step_split27 (bitwise the same results, whichever wave runs
+// it); the active cells of a wave stay within a window of one or two chunks, so node loads
+// remain close together.
 template <int INTERP>
 __device__ __forceinline__ void split27_refill(const float* __restrict__ fin, const float* __restrict__ min,
                                                const double* __restrict__ Tin, const float* __restrict__ g4in,
                                                float* __restrict__ fout, float* __restrict__ mout,
                                                double* __restrict__ Tout, int* __restrict__ niter_out,
-                                               const float* __restrict__ guess, const SplitArgs& a, int chunk) {
+                                               const float* __restrict__ guess, const SplitArgs& a, int chunk,
+                                               int* __restrict__ claim) {
   const Grid& g = a.g;
   const int lane = threadIdx.x;
   const int64_t ntiles = g.ntiles;
-  const int64_t w = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t t0 = w * chunk;
-  const int64_t total = (ntiles - t0 < chunk ? ntiles - t0 : chunk) * 64;  // cell slots of the chunk
+  const int64_t nchunks = (ntiles + chunk - 1) / chunk;
+  constexpr int kXcd = 8;
+  const int xk = (int)(blockIdx.x % kXcd);
+  const int64_t q = nchunks / kXcd, rq = nchunks % kXcd;
+  const int64_t c_lo = xk * q + (xk < rq ? xk : rq), c_hi = c_lo + q + (xk < rq ? 1 : 0);
   const NodeSrcF src = make_src(fin, g4in, g);
-  int64_t next = 0;  // first unstarted slot (the same in every lane)
+  int64_t t0 = 0, total = 0;  // first tile and cell slots of the current chunk
+  int64_t next = 0;           // first unstarted slot (the same in every lane)
+  bool more = true;           // the slab may still hold unclaimed chunks
+  if constexpr (!kRefillPersistent[INTERP]) {
+    // one chunk per workgroup, in the XCD-contiguous order of xcd_block
+    t0 = xcd_block(blockIdx.x, gridDim.x) * chunk;
+    total = (ntiles - t0 < chunk ? ntiles - t0 : chunk) * 64;
+    more = false;
+  }
   bool active = false;
   int64_t cell = 0;
   int x = 0, y = 0, z = 0;
   SplitIter st;
   while (true) {
+    const uint64_t want = __ballot(!active);
+    if (kRefillPersistent[INTERP] && want != 0 && next >= total && more) {
+      // lanes are free and the chunk has no unstarted cell: claim the slab's next chunk
+      int idx = 0;
+      if (lane == 0) idx = atomicAdd(&claim[xk], 1);
+      idx = __shfl(idx, 0);
+      const int64_t c = c_lo + idx;
+      if (c < c_hi) {
+        t0 = c * chunk;
+        total = (ntiles - t0 < chunk ? ntiles - t0 : chunk) * 64;
+        next = 0;
+      } else {
+        more = false;
+      }
+    }
     // lanes without a cell take the next slots, in lane order; a slot outside the grid (ragged
     // last tile) is used up without a cell
-    const uint64_t want = __ballot(!active);
     const int rank = __popcll(want & ((1ull << lane) - 1ull));
     if (!active) {
       const int64_t slot = next + rank;
@@ -1291,7 +1330,7 @@ __device__ __forceinline__ void split27_refill(const float* __restrict__ fin, co
       }
     }
     next += __popcll(want);
-    if (__ballot(active) == 0 && next >= total) break;
+    if (__ballot(active) == 0 && next >= total && !more) break;
     if (active) {
       // the loop condition of split27_iterate, one iteration per trip
       bool done = st.niter - st.base >= a.max_iter;
@@ -1309,8 +1348,8 @@ template <int INTERP>
 POND_SPLIT27_KERNEL void k_step_split27_refill(
     const float* __restrict__ fin, const float* __restrict__ min, const double* __restrict__ Tin,
     const float* __restrict__ g4in, float* __restrict__ fout, float* __restrict__ mout, double* __restrict__ Tout,
-    int* __restrict__ niter_out, const float* __restrict__ guess, const SplitArgs a, int chunk) {
-  split27_refill<INTERP>(fin, min, Tin, g4in, fout, mout, Tout, niter_out, guess, a, chunk);
+    int* __restrict__ niter_out, const float* __restrict__ guess, const SplitArgs a, int chunk, int* claim) {
+  split27_refill<INTERP>(fin, min, Tin, g4in, fout, mout, Tout, niter_out, guess, a, chunk, claim);
 }
 
 
@@ -1359,6 +1398,31 @@ POND_SPLIT27_KERNEL void k_step_split27_rest(const float* __restrict__ fin, cons
 
 __global__ void k_zero_count(int* c) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0;
+}
+
+__global__ void k_zero_ints(int* p, int n) {
+  if (blockIdx.x == 0 && (int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+// Workgroups of the persistent refill kernel: 16 64-lane blocks per CU, twice what two waves per
+// SIMD hold (blocks beyond those resident start when others have ended, find their slab used up
+// and end), a multiple of 8 (one share per XCD slab; at least 8 so that every slab has waves),
+// never more than the chunks, and one per chunk when there are fewer than 8.
+static unsigned refill_blocks(int64_t nchunks) {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    cus = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+           prop.multiProcessorCount > 0)
+              ? prop.multiProcessorCount
+              : 256;
+  }
+  if (nchunks < 8) return (unsigned)nchunks;
+  const int64_t resident = (int64_t)16 * cus;
+  int64_t b = resident < nchunks ? resident : nchunks;
+  b -= b % 8;
+  return (unsigned)(b < 8 ? 8 : b);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1642,10 +1706,14 @@ hipError_t launch_step_split(int lattice, int interp, const SplitPtrs& in, const
                        out.m, out.T, niter, pend, a);                                                             \
   } while (0)
   if (interp != 0 && a.refill_chunk > 0 && a.pass1_iters >= a.max_iter + kLearnedBudget) {
-    const int64_t waves = ((int64_t)blocks + a.refill_chunk - 1) / a.refill_chunk;
-#define POND_LAUNCH_R(I)                                                                                          \
-  hipLaunchKernelGGL(k_step_split27_refill<I>, dim3((unsigned)waves), dim3(64), 0, st, in.f, in.m, in.T, g4, out.f, \
-                     out.m, out.T, niter, guess, a, a.refill_chunk)
+    const int64_t nchunks = ((int64_t)blocks + a.refill_chunk - 1) / a.refill_chunk;
+    // the slabs' chunk counters, in the pending list's header (its count is the first int)
+    int* claim = pend.count + 16;
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, claim, 8);
+#define POND_LAUNCH_R(I)                                                                                              \
+  hipLaunchKernelGGL(k_step_split27_refill<I>, dim3(kRefillPersistent[I] ? refill_blocks(nchunks) : (unsigned)nchunks), \
+                     dim3(64), 0, st, in.f, in.m, in.T, g4,                                                              \
+                     out.f, out.m, out.T, niter, guess, a, a.refill_chunk, claim)
     if (interp == 2) POND_LAUNCH_R(2);
     else if (a.g.mirror) POND_LAUNCH_R(3);
     else POND_LAUNCH_R(1);
