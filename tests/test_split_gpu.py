@@ -164,9 +164,14 @@ def test_two_pass_split_is_independent_of_first_pass_length(gpu):
 def test_refill_matches_one_tile_per_wave(gpu):
     """The lane-refill single pass (a wave owns several 64-cell tiles and its lanes take new cells
     as theirs converge) gives bitwise the results of one tile per wave, on a shock-bubble state
-    and on a grid whose rows are shorter than a wave and whose last tile is ragged."""
+    and on a grid whose rows are shorter than a wave and whose last tile is ragged.  The WENO5C
+    waves are persistent and claim chunks from eight per-XCD slabs: the small grids give fewer
+    chunks than slabs (8 tiles: chunks of 3 make 3 chunks, one wave each) and slabs of unequal
+    length (9 tiles in chunks of 1: slab 0 holds two chunks, so its wave claims twice)."""
     cases = ((ic.shock_bubble(n=40).fields, (40, 40, 40), 4),
-             (smooth_random_state((37, 20, 11), seed=71, T0=0.01), (37, 20, 11), 3))
+             (smooth_random_state((37, 20, 11), seed=71, T0=0.01), (37, 20, 11), 3),
+             (smooth_random_state((16, 8, 4), seed=72, T0=0.01), (16, 8, 4), 3),
+             (smooth_random_state((24, 8, 3), seed=73, T0=0.01), (24, 8, 3), 3))
     for fields, shape, steps in cases:
         outs = []
         for chunk in (0, 1, 3, 8):
