@@ -262,21 +262,22 @@ def test_two_pass_fp64_is_independent_of_first_pass_length(gpu):
 def test_fp64_lane_refill_matches_oracle(gpu, interp):
     """The FP64 first pass with lane refill (a wave owns several 64-cell tiles and its lanes take
     new cells as theirs finish), with and without a second pass, gives the oracle's result bit for
-    bit, on a grid whose last tile is ragged."""
+    bit, on a grid whose last tile is ragged; so does the two-pass step with the node loads
+    through 64-bit global addresses (global_loads)."""
     shape = (37, 9, 7)
     f = smooth_random_state(shape, seed=72, T0=0.01)
     o = po.Oracle("D3Q27", shape, interp)
     o.set_state(*f)
     o.step(3)
     ref = o.fields()
-    for chunk, k1 in ((1, 8), (3, 2), (8, 1000)):
-        print(f"[refill f64] {interp} chunk {chunk} first pass {k1}", flush=True)
-        with Pond("D3Q27", shape, interp, "fp64", refill_chunk=chunk, pass1_iters=k1) as p:
+    for chunk, k1, gl in ((1, 8, 0), (3, 2, 0), (8, 1000, 0), (3, 2, 1)):
+        print(f"[refill f64] {interp} chunk {chunk} first pass {k1} global loads {gl}", flush=True)
+        with Pond("D3Q27", shape, interp, "fp64", refill_chunk=chunk, pass1_iters=k1, global_loads=gl) as p:
             p.set_state(*f)
             p.step(3)
             got = p.fields()
         for key in ("rho", "ux", "uy", "uz", "T", "niter"):
-            assert np.array_equal(got[key], ref[key]), (interp, chunk, k1, key)
+            assert np.array_equal(got[key], ref[key]), (interp, chunk, k1, gl, key)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "split"])
@@ -320,14 +321,15 @@ def _row_patch_grid(shape):
 def test_fp64_row_patch_order_matches_oracle(gpu, shape, interp, steps):
     """FP64 on a grid that takes the row-patch order and the default refill chunks and first-pass
     length (the configuration of every 256^3 and 384^3 run) equals the oracle bit for bit, and so
-    does plain row order with one tile per wave."""
+    do plain row order with one tile per wave and the node loads through 64-bit global addresses
+    (global_loads, the path of grids whose planes do not fit a 32-bit buffer offset)."""
     assert _row_patch_grid(shape)
     f = smooth_random_state(shape, seed=81, T0=0.01)
     o = po.Oracle("D3Q27", shape, interp)
     o.set_state(*f)
     o.step(steps)
     ref = o.fields()
-    for kw in ({}, {"tile_patch": 0, "refill_chunk": 0}):
+    for kw in ({}, {"tile_patch": 0, "refill_chunk": 0}, {"global_loads": 1}):
         print(f"[row patch f64] {shape} {interp} {kw}", flush=True)
         with Pond("D3Q27", shape, interp, "fp64", **kw) as p:
             p.set_state(*f)
