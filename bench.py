@@ -292,7 +292,7 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only; how the step would shard is described in DESIGN.md; "
+        print("bench.py: this build runs on one GPU only (the decomposition over several GPUs is not built); "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
     cfg = CONFIGS[args.config]

@@ -19,7 +19,7 @@
 // deterministic function of the gauge, so they converge with it.
 //
 // Starting gauge: the cell's previous gauge, or (learned initializer) a predicted gauge.  The
-// fixed-point iteration stays the judge: a prediction that is not finite or not positive in T
+// fixed-point iteration has the last word: a prediction that is not finite or not positive in T
 // is ignored, and if the residual stops falling before the noise floor, or the cell has not
 // converged after kLearnedBudget iterations, it restarts from its previous gauge.
 #include "pond_device.h"

@@ -1,6 +1,6 @@
 """GPU parity of the plain FP64 step against the CPU oracle, through the C ABI.
 
-The bar (north star): relative L-inf <= 1e-11 on rho, u, T after 100 steps on the same seeded
+The bar: relative L-inf <= 1e-11 on rho, u, T after 100 steps on the same seeded
 inputs; the plain path is built to reproduce the reference's rounding, so the tests also
 require identical per-cell iteration counts.
 """
