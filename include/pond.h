@@ -86,11 +86,9 @@ typedef struct {
                    their stencil halo in its L2; used where nx % 64 == 0 and tile_patch divides
                    ny and nz.  0: plain row order.  -1 (default): 8 for the POND_SPLIT
                    D3Q27 WENO5C step, 4 otherwise.
-     refill_chunk  64-cell tiles dealt to a wave at once, whose lanes take the next unstarted
-                   cell of the chunk when theirs converges (the split WENO5C step's waves are
-                   persistent: one whose chunk runs out takes the next chunk of its XCD's
-                   share).  0: one tile per wave.  -1 (default): FP64 2 tiles; split the whole
-                   x row with at least 4 tiles.
+     refill_chunk  64-cell tiles owned by one wave, whose lanes take the next unstarted cell
+                   of the chunk when theirs converges.  0: one tile per wave.  -1 (default):
+                   FP64 2 tiles; split the whole x row with at least 4 tiles.
      pass1_iters   iterations of the first pass of the two-pass step; cells not converged by
                    then are finished by a second kernel over a compacted list.  -1 (default):
                    FP64 8, split one pass.

@@ -1251,9 +1251,12 @@ POND_SPLIT27_KERNEL void k_step_split27(const float* __restrict__ fin, const flo
 }
 
 
-// Persistent refill waves (below) for WENO5C only: 384^3 shock-bubble step 157.8 -> 155.2 ms;
-// the Lagrange-4 step took 294 ms against 224 with one chunk per workgroup.
-constexpr bool kRefillPersistent[4] = {false, false, true, false};
+// Persistent refill waves (below) are built but not used: on the 384^3 shock-bubble they took
+// the WENO5C step from 157.8 to 155.2 ms, but the 256^3 vortex from 66.2 to 78.7 ms, the 384^3
+// Taylor-Green from 179.3 to 208.6 ms and the Lagrange-4 shock-bubble from 224 to 294 ms (a
+// wave handing over from one row to the next spans two distant rows, and the fetches from
+// beyond L2 double), so every kernel keeps one chunk per workgroup.
+constexpr bool kRefillPersistent[4] = {false, false, false, false};
 
 // Single pass with lane refill.  Cells are dealt in chunks of `chunk` consecutive 64-cell tiles
 // (a whole x row by default).  Whenever lanes of a wave finish their cells, they take the next
@@ -1262,10 +1265,12 @@ constexpr bool kRefillPersistent[4] = {false, false, true, false};
 // 1.3 while the slowest cell of a 64-cell tile needs 3.5-5, so lanes of the one-tile-per-wave
 // step sit idle most of the time.
 //
-// For WENO5C the waves are persistent: the grid is what the GPU holds at once, and a wave whose chunk has
-// no unstarted cell left claims the next chunk while its lanes still finish the old one, so the
-// tail of a chunk (the lanes of its last, slowest cells) is filled from the next one.  With one
-// chunk per wave, 30-40 % of the lane-iterations of the 384^3 shock-bubble step were such tails.
+// Each workgroup owns one chunk, in the XCD-contiguous order of xcd_block.  Where
+// kRefillPersistent is set (for no interpolator at present, see there) the waves are persistent
+// instead: the grid is what the GPU holds at once, and a wave whose chunk has no unstarted cell
+// left claims the next chunk while its lanes still finish the old one, so the tail of a chunk
+// (the lanes of its last, slowest cells) is filled from the next one.  With one chunk per wave,
+// 30-40 % of the lane-iterations of the 384^3 shock-bubble step are such tails.
 // The chunks are split into eight contiguous slabs, one per XCD (workgroups are dealt to the XCDs
 // in turn, so a wave's XCD is its block index mod 8), and the waves of XCD k claim the chunks of
 // slab k in order from the counter claim[k] (zeroed before the launch), so the waves resident on
